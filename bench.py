@@ -1,0 +1,158 @@
+"""Benchmark: string iterations per second of SCFT_string's solve_field path on one MI355X.
+
+Workload (the string-method configuration the metric is quoted on): 16 replicas on one GPU,
+64^3 grid, N = 128 contour steps, D = 6.0 R0, chiN = 18, f = 0.45, phidb = 0.9, canonical
+ensemble, both ends fixed, propagator and FFTs in FP32 with Q, free energy, norms, mixing and
+the spline in FP64. End replicas: lamellar W- = 0.5 cos(2 pi x / D) and a cylinder-like
+W- = 0.5 [cos X + cos(X/2 + sqrt3 Y/2) + cos(X/2 - sqrt3 Y/2)]/3; interior replicas are linear
+interpolants plus N(0, 0.01^2) noise (seed 2024); W+ = 0. A step is one solve_field iteration
+(scft.cu:592-806): W+ relaxation (<= 19 props) and one props per active replica, DEV, errors,
+perpendicular projection, mixing, arc length, spline redistribution, with FreeE at k % 50 == 0.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+CFG = dict(m=(64, 64, 64), D=(6.0, 6.0, 6.0), N=128, chi=18.0, f=0.45, phidb=0.9, ens=1, alpha=0.1,
+           nrep=16, fix=(1, 1))
+
+
+def initial_fields(m, D, R, seed=2024):
+    """End replicas: lamellar and cylinder-like W-; interior: linear interpolants + noise."""
+    M = int(np.prod(m))
+    xs = np.arange(m[0]) * D[0] / m[0]
+    ys = np.arange(m[1]) * D[1] / m[1]
+    X = 2 * np.pi * xs[:, None, None] / D[0] * np.ones(m)
+    Y = 2 * np.pi * ys[None, :, None] / D[1] * np.ones(m)
+    lam = 0.5 * np.cos(X)
+    cyl = 0.5 * (np.cos(X) + np.cos(X / 2 + np.sqrt(3) * Y / 2) + np.cos(X / 2 - np.sqrt(3) * Y / 2)) / 3
+    rng = np.random.default_rng(seed)
+    W = np.zeros((R, 2, M))
+    for i in range(R):
+        t = i / (R - 1)
+        W[i, 0] = ((1 - t) * lam + t * cyl).reshape(-1)
+        if 0 < i < R - 1:
+            W[i, 0] += 0.01 * rng.standard_normal(M)
+    return W
+
+
+def algorithmic_bytes_per_step(m, p):
+    """12 (M p + Mk 2p): 2 chains x 6 real 3-D FFTs, each reading its input and writing its
+    output once, pointwise factors fused (per contour step per replica)."""
+    M = m[0] * m[1] * m[2]
+    Mk = m[0] * m[1] * (m[2] // 2 + 1)
+    return 12 * (M * p + Mk * 2 * p)
+
+
+def cpu_baseline(m, D, N, R_active, props_per_iter, budget_s=20.0):
+    """Times the FP64 oracle (C, OpenMP) on a bounded sample: one props of one replica at the
+    full grid and contour length; scaled to string iterations/s of R_active * props_per_iter."""
+    from oracle import orc
+
+    threads = min(os.cpu_count() or 1, 16)
+    orc.set_threads(threads)
+    g = orc.Grid(m, D)
+    W = initial_fields(m, D, 3)[1]
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        g.props(W, N=N, alpha=CFG["alpha"], phic=CFG["phidb"], f=CFG["f"], ens=CFG["ens"])
+        n += 1
+        if time.perf_counter() - t0 > budget_s * 0.5 or n >= 3:
+            break
+    dt = (time.perf_counter() - t0) / n
+    return {"value": 1.0 / (dt * R_active * props_per_iter), "unit": "string iters/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} FP64 props of one replica at {m[0]}^3, N={N} ({dt:.2f} s each), scaled to "
+                      f"{R_active} replicas x {props_per_iter} props per iteration"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: all string replicas share it; "
+                                   "multi-GPU replica sharding is a design note only (DESIGN.md)",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    from scft_string_amd import Context
+
+    torch.cuda.init()
+    c = CFG
+    R = c["nrep"]
+    W0 = initial_fields(c["m"], c["D"], R)
+    ctx = Context(c["m"], c["D"], c["N"], c["f"], c["alpha"], c["ens"], c["phidb"], c["chi"], nrep=R,
+                  precision=a.precision, fix=c["fix"], dostring=1)
+    ctx.set_fields(W0)
+    print(f"bench: {R} replicas {c['m']} N={c['N']} {a.precision}; warmup {a.warmup}", file=sys.stderr, flush=True)
+
+    def iteration(k):
+        rep = ctx.string_iterate(k)
+        if k % 50 == 0:
+            ctx.free_energy()
+        return rep
+
+    for k in range(a.warmup):
+        iteration(k)
+    torch.cuda.synchronize()
+    print("bench: timing", a.steps, "iterations", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    props = 0
+    for k in range(a.warmup, a.warmup + a.steps):
+        rep = iteration(k)
+        props += int(np.sum(rep["steps"]))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    iters_per_s = a.steps / dt
+
+    # roofline of the fused contour step (4 launches per step over all active replicas)
+    n_act = R - sum(c["fix"])
+    ms_step = ctx.time_props(n_act, reps=2)
+    p = 4 if a.precision == "fp32" else 8
+    bstep = algorithmic_bytes_per_step(c["m"], p) * n_act
+    achieved = bstep / (ms_step * 1e-3) / 1e9
+    fe = ctx.free_energy()
+    ctx.close()
+
+    out = {
+        "metric": "string iters/sec 16x64^3xN128 (batched FP32 propagator, FP64 field update)",
+        "value": iters_per_s, "unit": "string iters/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": dt / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if a.precision == "fp32" else "f64", "data": "synthetic (seeded lamellar/cylinder string)",
+        "config": {"workload": "string method, 16 replicas x 64^3, N=128, D=6.0, chiN=18, f=0.45, phidb=0.9, "
+                               "canonical, fix=(1,1)", "replicas": R, "grid": list(c["m"]), "N": c["N"],
+                   "props_per_iteration": props / a.steps, "parallelism": "1 GPU, all replicas batched"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "contour step (column, plane_mid, column, plane_end) over "
+                               f"{n_act} replicas", "ms_per_contour_step": ms_step},
+        "free_energy": {"FE": fe["FE"], "FEs": list(fe["FEs"])},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(c["m"], c["D"], c["N"], n_act, props / a.steps / n_act,
+                                           a.cpu_budget)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,131 @@
+/*
+ * scft_api.h -- C-ABI of the MI355X (gfx950) SCFT string-method library (libscft_hip.so).
+ *
+ * Drop-in boundary for SCFT_string's solve_field hot path. The reference binds this path
+ * through the C++ class `density` (density.h:225-548) and the host solvers of scft.cu;
+ * each entry point below names the reference interface it replaces.
+ *
+ * Conventions
+ *  - Every call returns SCFT_OK (0) or an error code; scft_last_error() gives the message.
+ *    Nothing in the library calls exit() (the reference's HANDLE_ERROR does,
+ *    GPU_kernels.h:16-48).
+ *  - Host arrays at the boundary are FP64 whatever the internal precision.
+ *  - Field layout per replica (scft.cu:1037-1051): W row = [W- | W+] (2M doubles),
+ *    densities = [phiA | phiB | phih | phicB] (4M doubles); grid index
+ *    r = z + y*m2 + x*m2*m1 (scft.cu:462-464).
+ *  - One context per host thread; all work is queued on the context's HIP stream and
+ *    every call that returns host data synchronises that stream.
+ */
+#ifndef SCFT_API_H
+#define SCFT_API_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum scft_status {
+    SCFT_OK = 0,
+    SCFT_ERR_INVALID = 1,     /* bad argument or parameter */
+    SCFT_ERR_UNSUPPORTED = 2, /* grid size / precision combination without a kernel */
+    SCFT_ERR_HIP = 3,         /* HIP runtime error */
+    SCFT_ERR_NOMEM = 4,       /* device allocation failed */
+    SCFT_ERR_NAN = 5          /* a field error norm became NaN (scft.cu:539, 770 assert) */
+} scft_status;
+
+typedef enum scft_precision {
+    SCFT_FP64 = 0, /* propagator and FFTs in FP64 */
+    SCFT_FP32 = 1  /* propagator and FFTs in FP32; Q, norms, mixing, spline stay FP64 */
+} scft_precision;
+
+/* Run parameters: the keys of the reference's input.dat (scft.cu:120-251) plus the
+ * compile-time replica count strn (scft.cu:35), made a runtime value here. */
+typedef struct scft_params {
+    int m[3];        /* grid points, x slowest */
+    double D[3];     /* box lengths in R0 */
+    int N;           /* contour steps of the copolymer */
+    double f;        /* A fraction as read; rounded to round(N f)/N (scft.cu:1008) */
+    double alpha;    /* homopolymer length / N (scft.cu:38) */
+    int ens;         /* 1 canonical, 2 grand canonical */
+    double phidb;    /* copolymer volume fraction (ens 1) or fugacity (ens 2) */
+    double chi;      /* chi N */
+    int nrep;        /* replicas on the string (strn) */
+    int precision;   /* scft_precision */
+    int fix[2];      /* fix first / last replica */
+    int dostring;    /* 1: string method, 0: independent replicas */
+    int justFE;      /* 1: only the free energy (main skips solve_field) */
+    int springsteps; /* iterations with the spring term before redistribution starts */
+    int device;      /* HIP device ordinal */
+} scft_params;
+
+typedef struct scft_ctx scft_ctx;
+
+/* Replaces `new density(N, D)` + setupSplineBuffers (scft.cu:1010-1011, density.h:272-315,
+ * 462-476) and the cuFFT plans of step::step (density.h:41-75). Allocates all device
+ * memory for nrep replicas. */
+scft_status scft_ctx_create(const scft_params *p, scft_ctx **out);
+scft_status scft_ctx_destroy(scft_ctx *ctx);
+const char *scft_last_error(const scft_ctx *ctx);
+scft_status scft_sync(scft_ctx *ctx);
+
+/* Replaces density::props (density.h:344-459), batched over `nrep` independent rows:
+ * W [nrep][2M] in; phi [nrep][4M], lnQ [nrep], phis [nrep][5] out (phi, phis may be NULL).
+ * Uses scratch state only; the context's field state (scft_set_fields) is untouched. */
+scft_status scft_props_batch(scft_ctx *ctx, const double *W, double *phi, double *lnQ,
+                             double *phis, int nrep);
+
+/* Same as scft_props_batch with device pointers for W (row stride 2M) and phi
+ * (row stride 4M); lnQ and phis are host arrays. */
+scft_status scft_props_batch_device(scft_ctx *ctx, const double *dW, double *dphi,
+                                    double *lnQ, double *phis, int nrep);
+
+/* Context field state: W rows [nrep][2M] (the W-, W+ part of scft.cu's W[ii]). */
+scft_status scft_set_fields(scft_ctx *ctx, const double *W);
+scft_status scft_get_fields(scft_ctx *ctx, double *W);
+/* Densities of the last props per replica, [nrep][4M] (W[ii][2M:6M] in scft.cu). */
+scft_status scft_get_densities(scft_ctx *ctx, double *phi);
+/* Tangent dW-/dalpha of the last projection, [nrep][M] (W[ii][6M:7M] in scft.cu). */
+scft_status scft_get_tangent(scft_ctx *ctx, double *dWda);
+
+/* Replaces solve_field0 (scft.cu:516-554) for every replica with which[ii] != 0 (NULL:
+ * all), batched, each replica with its own mixing parameter and early exit.
+ * k_out[ii] receives the value solve_field0 returns (may be NULL). */
+scft_status scft_solve_w_plus(scft_ctx *ctx, int maxIter, double errTol, const int *which,
+                              int *k_out);
+
+/* Per-iteration report of one solve_field body (scft.cu:592-757). Arrays hold nrep
+ * entries; the caller provides them (any may be NULL). */
+typedef struct scft_iter_report {
+    double *errs, *errs3, *x, *xref, *dotWs, *dotWs2, *normDER, *normDEV;
+    int *steps;
+    double fitup;
+    double stepav;
+} scft_iter_report;
+
+/* One iteration k of solve_field (scft.cu:592-757) for the single-process layout:
+ * solve_field0(20, errTol) + props for every active replica, DEV, errs, perpendicular
+ * projection, fixed-lambda mixing, arc length, spline redistribution. Fixed ends
+ * (params.fix) are excluded as in scft.cu:587-588. alf [nrep] receives the arc-length
+ * coordinate (scft.cu:737); may be NULL. */
+scft_status scft_string_iterate(scft_ctx *ctx, int k, double errTol, double *alf,
+                                scft_iter_report *rep);
+
+/* Replaces FreeE (scft.cu:812-853), single process: solve_field0(100, 1e-4) and props for
+ * every active replica plus both ends, then FEs0 = -lnQ, FEs1, FEs, phctot, alf.
+ * Any output may be NULL. */
+scft_status scft_free_energy(scft_ctx *ctx, double *FE, double *FEs, double *FEs0,
+                             double *FEs1, double *phctot, double *alf);
+
+/* Diagnostics for tests: after a props call on slot `slot` of the scratch batch,
+ * qq[s] = <q1'[s] q2'[N-s]> for s = 0..N (the A4 symmetry invariant, with the
+ * mean-field factoring left in; all entries equal QC' in exact arithmetic). */
+scft_status scft_debug_qq_invariant(scft_ctx *ctx, int slot, double *qq);
+
+/* Diagnostics for the benchmark: runs props on the first nrep rows of the context field
+ * state `reps` times and returns the mean time per contour step (ms), measured with HIP
+ * events on the context stream around the contour loop (begin + N steps). */
+scft_status scft_debug_time_props(scft_ctx *ctx, int nrep, int reps, double *ms_per_step);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

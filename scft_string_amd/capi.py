@@ -1,0 +1,209 @@
+"""ctypes binding of libscft_hip.so (include/scft_api.h).
+
+The product path is the HIP library; there is no CPU fallback. Loading raises if the
+library has not been built (python -m scft_string_amd.build).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libscft_hip.so")
+
+SCFT_OK, SCFT_ERR_INVALID, SCFT_ERR_UNSUPPORTED, SCFT_ERR_HIP, SCFT_ERR_NOMEM, SCFT_ERR_NAN = range(6)
+SCFT_FP64, SCFT_FP32 = 0, 1
+
+EXPORTED = [
+    "scft_ctx_create", "scft_ctx_destroy", "scft_last_error", "scft_sync", "scft_props_batch",
+    "scft_props_batch_device", "scft_set_fields", "scft_get_fields", "scft_get_densities",
+    "scft_get_tangent", "scft_solve_w_plus", "scft_string_iterate", "scft_free_energy",
+    "scft_debug_qq_invariant", "scft_debug_time_props",
+]
+
+
+class ScftError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"scft status {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("m", C.c_int * 3), ("D", C.c_double * 3), ("N", C.c_int), ("f", C.c_double),
+        ("alpha", C.c_double), ("ens", C.c_int), ("phidb", C.c_double), ("chi", C.c_double),
+        ("nrep", C.c_int), ("precision", C.c_int), ("fix", C.c_int * 2), ("dostring", C.c_int),
+        ("justFE", C.c_int), ("springsteps", C.c_int), ("device", C.c_int),
+    ]
+
+
+class IterReport(C.Structure):
+    _fields_ = [
+        ("errs", C.POINTER(C.c_double)), ("errs3", C.POINTER(C.c_double)), ("x", C.POINTER(C.c_double)),
+        ("xref", C.POINTER(C.c_double)), ("dotWs", C.POINTER(C.c_double)), ("dotWs2", C.POINTER(C.c_double)),
+        ("normDER", C.POINTER(C.c_double)), ("normDEV", C.POINTER(C.c_double)), ("steps", C.POINTER(C.c_int)),
+        ("fitup", C.c_double), ("stepav", C.c_double),
+    ]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m scft_string_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+        L.scft_ctx_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
+        L.scft_ctx_destroy.argtypes = [vp]
+        L.scft_last_error.argtypes = [vp]
+        L.scft_last_error.restype = C.c_char_p
+        L.scft_sync.argtypes = [vp]
+        L.scft_props_batch.argtypes = [vp, dp, dp, dp, dp, C.c_int]
+        L.scft_props_batch_device.argtypes = [vp, C.c_void_p, C.c_void_p, dp, dp, C.c_int]
+        L.scft_set_fields.argtypes = [vp, dp]
+        L.scft_get_fields.argtypes = [vp, dp]
+        L.scft_get_densities.argtypes = [vp, dp]
+        L.scft_get_tangent.argtypes = [vp, dp]
+        L.scft_solve_w_plus.argtypes = [vp, C.c_int, C.c_double, ip, ip]
+        L.scft_string_iterate.argtypes = [vp, C.c_int, C.c_double, dp, C.POINTER(IterReport)]
+        L.scft_free_energy.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+        L.scft_debug_qq_invariant.argtypes = [vp, C.c_int, dp]
+        L.scft_debug_time_props.argtypes = [vp, C.c_int, C.c_int, dp]
+        for name in EXPORTED:
+            if name not in ("scft_last_error",):
+                getattr(L, name).restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """One scft_ctx: grid, chain parameters, nrep replicas' device state."""
+
+    def __init__(self, m, D, N, f, alpha, ens, phidb, chi, nrep=1, precision="fp64", fix=(0, 0),
+                 dostring=1, justFE=0, springsteps=0, device=0):
+        self.m = tuple(int(v) for v in m)
+        self.M = int(np.prod(self.m))
+        self.nrep = int(nrep)
+        self.N = int(N)
+        prec = {"fp64": SCFT_FP64, "fp32": SCFT_FP32}[precision]
+        p = Params(m=(C.c_int * 3)(*self.m), D=(C.c_double * 3)(*[float(v) for v in D]), N=int(N), f=float(f),
+                   alpha=float(alpha), ens=int(ens), phidb=float(phidb), chi=float(chi), nrep=self.nrep,
+                   precision=prec, fix=(C.c_int * 2)(*fix), dostring=int(dostring), justFE=int(justFE),
+                   springsteps=int(springsteps), device=int(device))
+        self.params = p
+        h = C.c_void_p()
+        rc = lib().scft_ctx_create(C.byref(p), C.byref(h))
+        if rc != SCFT_OK:
+            raise ScftError(rc, f"scft_ctx_create failed for grid {self.m} ({precision})")
+        self._h = h
+
+    def _chk(self, rc):
+        if rc != SCFT_OK:
+            raise ScftError(rc, lib().scft_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().scft_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---------------------------------------------------------------- props (density.h:344)
+    def props_batch(self, W):
+        """W: (n, 2, M) or (n, 2M). Returns phi (n, 4, M), lnQ (n,), phis (n, 5)."""
+        W = np.ascontiguousarray(W, dtype=np.float64).reshape(-1, 2 * self.M)
+        n = W.shape[0]
+        phi = np.empty((n, 4 * self.M))
+        lnQ = np.empty(n)
+        phis = np.empty((n, 5))
+        self._chk(lib().scft_props_batch(self._h, _dp(W), _dp(phi), _dp(lnQ), _dp(phis), n))
+        return phi.reshape(n, 4, self.M), lnQ, phis
+
+    def props_batch_device(self, W_dev, phi_dev, n):
+        """Device-pointer variant; W_dev, phi_dev are torch CUDA float64 tensors."""
+        lnQ = np.empty(n)
+        phis = np.empty((n, 5))
+        self._chk(lib().scft_props_batch_device(self._h, C.c_void_p(W_dev.data_ptr()),
+                                                C.c_void_p(phi_dev.data_ptr()), _dp(lnQ), _dp(phis), n))
+        return lnQ, phis
+
+    def qq_invariant(self, slot=0):
+        out = np.empty(self.N + 1)
+        self._chk(lib().scft_debug_qq_invariant(self._h, int(slot), _dp(out)))
+        return out
+
+    def time_props(self, nrep, reps=3):
+        """mean ms per contour step of props over the first nrep field-state rows"""
+        out = C.c_double()
+        self._chk(lib().scft_debug_time_props(self._h, int(nrep), int(reps), C.byref(out)))
+        return out.value
+
+    # ---------------------------------------------------------------- field state
+    def set_fields(self, W):
+        W = np.ascontiguousarray(W, dtype=np.float64).reshape(self.nrep, 2 * self.M)
+        self._chk(lib().scft_set_fields(self._h, _dp(W)))
+
+    def get_fields(self):
+        W = np.empty((self.nrep, 2 * self.M))
+        self._chk(lib().scft_get_fields(self._h, _dp(W)))
+        return W
+
+    def get_densities(self):
+        phi = np.empty((self.nrep, 4 * self.M))
+        self._chk(lib().scft_get_densities(self._h, _dp(phi)))
+        return phi
+
+    def get_tangent(self):
+        t = np.empty((self.nrep, self.M))
+        self._chk(lib().scft_get_tangent(self._h, _dp(t)))
+        return t
+
+    # ---------------------------------------------------------------- solvers (scft.cu)
+    def solve_w_plus(self, maxIter, errTol, which=None):
+        k = np.zeros(self.nrep, dtype=np.int32)
+        wp = None
+        if which is not None:
+            which = np.ascontiguousarray(which, dtype=np.int32)
+            wp = which.ctypes.data_as(C.POINTER(C.c_int))
+        self._chk(lib().scft_solve_w_plus(self._h, int(maxIter), float(errTol), wp,
+                                          k.ctypes.data_as(C.POINTER(C.c_int))))
+        return k
+
+    def string_iterate(self, k, errTol=1e-3):
+        R = self.nrep
+        arrs = {n: np.zeros(R) for n in ("errs", "errs3", "x", "xref", "dotWs", "dotWs2", "normDER", "normDEV")}
+        steps = np.zeros(R, dtype=np.int32)
+        alf = np.zeros(R)
+        rep = IterReport(**{n: _dp(a) for n, a in arrs.items()}, steps=steps.ctypes.data_as(C.POINTER(C.c_int)))
+        self._chk(lib().scft_string_iterate(self._h, int(k), float(errTol), _dp(alf), C.byref(rep)))
+        out = dict(arrs)
+        out["steps"] = steps
+        out["alf"] = alf
+        out["fitup"] = rep.fitup
+        out["stepav"] = rep.stepav
+        return out
+
+    def free_energy(self):
+        R = self.nrep
+        FE = C.c_double()
+        FEs, FEs0, FEs1, ph, alf = (np.zeros(R) for _ in range(5))
+        self._chk(lib().scft_free_energy(self._h, C.byref(FE), _dp(FEs), _dp(FEs0), _dp(FEs1), _dp(ph), _dp(alf)))
+        return {"FE": FE.value, "FEs": FEs, "FEs0": FEs0, "FEs1": FEs1, "phctot": ph, "alf": alf}

@@ -1,0 +1,53 @@
+// field_kernels.hpp -- FP64 field update of the string method on the device
+// (scft.cu:516-853). All replicas' fields live in one [R][2M] array; reductions are two-stage
+// (block partials in a fixed order) so results are deterministic run to run.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace scft {
+
+constexpr int FK_NT = 256;
+constexpr int SPLINE_RMAX = 64;  // replicas supported by the per-point spline kernels
+
+// solve_field0 body (scft.cu:527-543) for nb slots: DEV- = 0, DEV+ = chi (phiB + phiA - 1),
+// W += lambda DEV, partial sums of DEV^2. part layout [slot][nblk].
+void launch_wplus_mix(const int* d_slot2rep, int nb, long long M, double chi, const double* d_lambda,
+                      const double* phi, double* W, double* DEV, double* part, int nblk, hipStream_t st);
+
+// solve_field DEV (scft.cu:604-608) for nb slots; spring term when spring != 0.
+// part [slot][nblk][2] = (sum DEV^2 over 2M, sum DEV-^2 over M)
+void launch_dev_full(const int* d_slot2rep, int nb, int R, long long M, double chi, int spring, double eps,
+                     const double* phi, const double* W, double* DEV, double* part, int nblk, hipStream_t st);
+
+// partial sums of (W-[i+1] - W-[i])^2 for i < R-1 (scft.cu:351-355); part [i][nblk]
+void launch_edist(int R, long long M, const double* W, double* part, int nblk, hipStream_t st);
+
+// finalize: out[slot*ncomp + c] = sum over nblk of part[(slot*nblk + b)*ncomp + c]
+void launch_finalize(const double* part, int nslots, int nblk, int ncomp, double* out, hipStream_t st);
+
+// spline derivative dW-/dalpha at the nodes (GPU_kernels.h:232-306) for every r; writes
+// dWda[ii] for active[ii]
+void launch_spline_deriv(int R, long long M, const double* d_x, const int* d_active, const double* W,
+                         double* dWda, hipStream_t st);
+
+// part [slot][nblk][2] = (sum dWda^2, sum DEV- dWda) per slot (scft.cu:649-667)
+void launch_tangent_dots(const int* d_slot2rep, int nb, long long M, const double* dWda, const double* DEV,
+                         double* part, int nblk, hipStream_t st);
+
+// projection (scft.cu:670-674) + fixed-lambda mixing (scft.cu:690-693) for every replica:
+// DEV- -= cproj[ii] dWda; W += lambda DEV; part [ii][nblk][2] = (sum DEV'^2 over 2M,
+// sum DEV-' dWda)
+void launch_project_mix(int R, long long M, const double* d_cproj, double lambda, const double* dWda,
+                        double* DEV, double* W, double* part, int nblk, hipStream_t st);
+
+// redistribution (scft.cu:745-757, density.h:511-528): for interior active replicas,
+// W- += (spline_x(W-)(xref) - W-) lambda
+void launch_redistribute(int R, long long M, const double* d_x, const double* d_xref, const int* d_active,
+                         double lambda, double* W, hipStream_t st);
+
+// FreeE sums (scft.cu:823-828) for nb slots: part [slot][nblk][2] =
+// (sum chi phiA phiB - wa phiA - wb phiB, sum phiA)
+void launch_fe_sums(const int* d_slot2rep, int nb, long long M, double chi, const double* W, const double* phi,
+                    double* part, int nblk, hipStream_t st);
+
+}  // namespace scft

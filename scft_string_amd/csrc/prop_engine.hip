@@ -1,0 +1,259 @@
+// prop_engine.hip -- kernel instantiation table and the batched props driver.
+#include "prop_engine.hpp"
+#include "prop_kernels.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+namespace scft {
+
+// ---------------------------------------------------------------- per-grid kernel sets
+template <typename T, int N0, int N1, int N2>
+struct KS {
+    static constexpr int NE = N1 * N2;
+    static constexpr int NTP = NE <= 256 ? 64 : NE <= 1024 ? 128 : NE <= 4096 ? 256 : NE <= 8192 ? 512 : 1024;
+    static constexpr int TC = 64;
+    static constexpr int NTC = 256;
+    static constexpr size_t lds_plane = (size_t)(N1 * PlaneGeom<N1, N2>::P2 + N1 + N2) * sizeof(cpx<T>);
+    static constexpr size_t lds_col = (size_t)(N0 * TC + N0) * sizeof(cpx<T>) + (size_t)N0 * sizeof(T);
+    static constexpr int ntile = (N1 * N2 + TC - 1) / TC;
+    static_assert(lds_plane <= 160 * 1024, "plane does not fit in LDS");
+    static_assert(lds_col <= 160 * 1024, "column tile does not fit in LDS");
+    static_assert(fft_len_supported(N0) && fft_len_supported(N1) && fft_len_supported(N2), "radix");
+
+    static hipError_t setup()
+    {
+        hipError_t e;
+        e = hipFuncSetAttribute((const void*)k_plane_end<T, N1, N2, NTP>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_plane);
+        if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute((const void*)k_plane_mid<T, N1, N2, NTP>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_plane);
+        if (e != hipSuccess) return e;
+        return hipFuncSetAttribute((const void*)k_column<T, N0, N1, N2, TC, NTC>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_col);
+    }
+    static void begin(const PropArgs<T>& a, hipStream_t st)
+    {
+        hipLaunchKernelGGL((k_plane_end<T, N1, N2, NTP>), dim3(N0, a.nb), dim3(NTP), lds_plane, st, a, 0,
+                           (int)PL_BEGIN);
+    }
+    static void step(const PropArgs<T>& a, int s, hipStream_t st)
+    {
+        hipLaunchKernelGGL((k_column<T, N0, N1, N2, TC, NTC>), dim3(ntile, 2 * a.nb), dim3(NTC), lds_col, st, a,
+                           a.nb, a.nb);
+        hipLaunchKernelGGL((k_plane_mid<T, N1, N2, NTP>), dim3(N0, a.nb), dim3(NTP), lds_plane, st, a, s);
+        hipLaunchKernelGGL((k_column<T, N0, N1, N2, TC, NTC>), dim3(ntile, a.nb), dim3(NTC), lds_col, st, a,
+                           a.nb, 0);
+        hipLaunchKernelGGL((k_plane_end<T, N1, N2, NTP>), dim3(N0, a.nb), dim3(NTP), lds_plane, st, a, s,
+                           (int)(s == a.N - 1 ? PL_LAST : PL_STEP));
+    }
+    static KernelSet<T> make() { return KernelSet<T>{N0, N1, N2, &setup, &begin, &step}; }
+};
+
+#define SCFT_GRID_COMMON(T)                                                                         \
+    KS<T, 8, 8, 8>::make(), KS<T, 12, 12, 12>::make(), KS<T, 16, 16, 16>::make(),                   \
+        KS<T, 24, 24, 24>::make(), KS<T, 32, 32, 32>::make(), KS<T, 48, 48, 48>::make(),            \
+        KS<T, 64, 64, 64>::make(), KS<T, 96, 96, 96>::make(), KS<T, 12, 8, 16>::make(),             \
+        KS<T, 8, 12, 10>::make(), KS<T, 16, 12, 20>::make()
+
+template <>
+const KernelSet<float>* find_kernels<float>(int m0, int m1, int m2)
+{
+    static const KernelSet<float> table[] = {SCFT_GRID_COMMON(float), KS<float, 128, 128, 128>::make()};
+    for (const auto& k : table)
+        if (k.m0 == m0 && k.m1 == m1 && k.m2 == m2) return &k;
+    return nullptr;
+}
+
+template <>
+const KernelSet<double>* find_kernels<double>(int m0, int m1, int m2)
+{
+    static const KernelSet<double> table[] = {SCFT_GRID_COMMON(double)};
+    for (const auto& k : table)
+        if (k.m0 == m0 && k.m1 == m1 && k.m2 == m2) return &k;
+    return nullptr;
+}
+
+std::string supported_grids(bool fp64)
+{
+    std::string s = "8^3 12^3 16^3 24^3 32^3 48^3 64^3 96^3 12x8x16 8x12x10 16x12x20";
+    if (!fp64) s += " 128^3";
+    return s;
+}
+
+// ---------------------------------------------------------------- PropEngine
+template <typename T>
+PropEngine<T>::~PropEngine()
+{
+    void* ptrs[] = {Ch_, Cf_, e4A_, e4B_, qs_, tw_[0], tw_[1], tw_[2], g_[0], g_[1], g_[2], part_, cAB_, scal_};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (hipEvent_t e : ev_)
+        if (e) (void)hipEventDestroy(e);
+}
+
+template <typename T>
+hipError_t PropEngine<T>::enable_timing(bool on)
+{
+    timing_ = on;
+    for (int i = 0; i < 2 && on; ++i)
+        if (!ev_[i]) {
+            hipError_t e = hipEventCreate(&ev_[i]);
+            if (e != hipSuccess) return e;
+        }
+    return hipSuccess;
+}
+
+template <typename T>
+double PropEngine<T>::loop_ms() const
+{
+    float ms = 0;
+    if (!ev_[0] || !ev_[1] || hipEventElapsedTime(&ms, ev_[0], ev_[1]) != hipSuccess) return -1.0;
+    return (double)ms;
+}
+
+template <typename T>
+static hipError_t upload_twiddles(cpx<T>** dst, int n)
+{
+    std::vector<cpx<T>> h(n);
+    const long double pi = acosl(-1.0L);
+    for (int t = 0; t < n; ++t) {
+        const long double a = -2.0L * pi * (long double)t / (long double)n;
+        h[t].x = (T)cosl(a);
+        h[t].y = (T)sinl(a);
+    }
+    hipError_t e = hipMalloc((void**)dst, sizeof(cpx<T>) * n);
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*dst, h.data(), sizeof(cpx<T>) * n, hipMemcpyHostToDevice);
+}
+
+template <typename T>
+std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nbmax)
+{
+    for (int i = 0; i < 3; ++i) { m_[i] = m[i]; D_[i] = D[i]; }
+    N_ = N;
+    nbmax_ = nbmax;
+    M_ = (long long)m[0] * m[1] * m[2];
+    ks_ = find_kernels<T>(m[0], m[1], m[2]);
+    if (!ks_) {
+        char b[256];
+        snprintf(b, sizeof b, "no %s propagator kernels for grid %dx%dx%d (supported: %s)",
+                 sizeof(T) == 8 ? "FP64" : "FP32", m[0], m[1], m[2], supported_grids(sizeof(T) == 8).c_str());
+        return b;
+    }
+    hipError_t e = ks_->setup();
+    if (e != hipSuccess) return std::string("hipFuncSetAttribute: ") + hipGetErrorString(e);
+    nblk_ = (int)std::min<long long>(1024, (M_ + RED_NT - 1) / RED_NT);
+    auto alloc = [&](void** p, size_t bytes) -> bool {
+        if (hipMalloc(p, bytes) != hipSuccess) return false;
+        bytes_ += bytes;
+        return true;
+    };
+    const size_t fld = sizeof(cpx<T>) * (size_t)M_ * nbmax;
+    if (!alloc((void**)&Ch_, fld) || !alloc((void**)&Cf_, fld) ||
+        !alloc((void**)&e4A_, sizeof(T) * (size_t)M_ * nbmax) ||
+        !alloc((void**)&e4B_, sizeof(T) * (size_t)M_ * nbmax) ||
+        !alloc((void**)&qs_, sizeof(cpx<T>) * (size_t)M_ * (size_t)(N + 1) * nbmax) ||
+        !alloc((void**)&part_, sizeof(double) * (size_t)nblk_ * 5 * nbmax) ||
+        !alloc((void**)&cAB_, sizeof(double) * 2 * nbmax) || !alloc((void**)&scal_, sizeof(double) * 8 * nbmax))
+        return "device allocation failed (propagator buffers)";
+    for (int i = 0; i < 3; ++i) {
+        e = upload_twiddles<T>(&tw_[i], m[i]);
+        if (e != hipSuccess) return std::string("twiddles: ") + hipGetErrorString(e);
+        if (hipMalloc((void**)&g_[i], sizeof(T) * m[i]) != hipSuccess) return "device allocation failed (tables)";
+    }
+    e = set_box(D);
+    if (e != hipSuccess) return std::string("kinetic tables: ") + hipGetErrorString(e);
+    return "";
+}
+
+// g_i[k] = exp(-(2 pi k_i / D_i)^2 ds / 12) with k_i the signed frequency of index k
+// (setup_k_sq, density.h:79-95); the product over axes is exp(-k^2 ds/12).
+template <typename T>
+hipError_t PropEngine<T>::set_box(const double D[3])
+{
+    const double pi = 4 * atan(1.0);
+    const double ds = 1.0 / N_;
+    for (int i = 0; i < 3; ++i) {
+        D_[i] = D[i];
+        std::vector<T> h(m_[i]);
+        for (int k = 0; k < m_[i]; ++k) {
+            const int kk = (k <= m_[i] / 2) ? k : k - m_[i];
+            const double k2 = 4 * pi * pi * (double)kk * kk / (D[i] * D[i]);
+            h[k] = (T)exp(-k2 * ds / 12);
+        }
+        hipError_t e = hipMemcpy(g_[i], h.data(), sizeof(T) * m_[i], hipMemcpyHostToDevice);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+template <typename T>
+hipError_t PropEngine<T>::run(const double* dW, long long wstride, double* dphi, long long phistride,
+                              const int* d_slot2rep, int nb, const ChainConst& cc, double* d_phis, hipStream_t st)
+{
+    if (nb <= 0) return hipSuccess;
+    PropArgs<T> a;
+    a.M = M_;
+    a.N = cc.N;
+    a.n1 = cc.n1;
+    a.n2 = cc.n2;
+    a.Nh = cc.Nh;
+    a.nb = nb;
+    a.Ch = Ch_;
+    a.Cf = Cf_;
+    a.e4A = e4A_;
+    a.e4B = e4B_;
+    a.qs = qs_;
+    a.tw0 = tw_[0];
+    a.tw1 = tw_[1];
+    a.tw2 = tw_[2];
+    a.gx = g_[0];
+    a.gy = g_[1];
+    a.gz = g_[2];
+    a.invM = (T)(1.0 / (double)M_);
+
+    const dim3 rg(nblk_, nb);
+    hipLaunchKernelGGL(k_wsum, rg, dim3(RED_NT), 0, st, dW, wstride, d_slot2rep, M_, part_);
+    hipLaunchKernelGGL(k_wmean, dim3(nb), dim3(RED_NT), 0, st, part_, nblk_, M_, cAB_);
+    hipLaunchKernelGGL((k_factors<T>), rg, dim3(RED_NT), 0, st, dW, wstride, d_slot2rep, M_, cc.ds, cAB_, e4A_,
+                       e4B_);
+    if (timing_) (void)hipEventRecord(ev_[0], st);
+    ks_->begin(a, st);
+    for (int s = 0; s < cc.N; ++s) ks_->step(a, s, st);
+    if (timing_) (void)hipEventRecord(ev_[1], st);
+    hipLaunchKernelGGL((k_qsum<T>), rg, dim3(RED_NT), 0, st, qs_, M_, cc.N, cc.Nh, part_);
+    hipLaunchKernelGGL(k_scalars, dim3(nb), dim3(RED_NT), 0, st, part_, nblk_, M_, cc, cAB_, scal_);
+    hipLaunchKernelGGL((k_density<T>), rg, dim3(RED_NT), 0, st, qs_, M_, cc, scal_, dphi, phistride, d_slot2rep,
+                       part_);
+    hipLaunchKernelGGL(k_phis, dim3(nb), dim3(RED_NT), 0, st, part_, nblk_, M_, d_phis);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t PropEngine<T>::qq_invariant(int slot, double* qq, hipStream_t st)
+{
+    std::vector<double> buf((size_t)(N_ + 1) * nblk_);
+    double* dtmp = nullptr;
+    hipError_t e = hipMalloc((void**)&dtmp, sizeof(double) * buf.size());
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_qq<T>), dim3(nblk_, N_ + 1), dim3(RED_NT), 0, st, qs_, M_, N_, slot, dtmp);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(buf.data(), dtmp, sizeof(double) * buf.size(), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(dtmp);
+    if (e != hipSuccess) return e;
+    for (int s = 0; s <= N_; ++s) {
+        double v = 0;
+        for (int b = 0; b < nblk_; ++b) v += buf[(size_t)s * nblk_ + b];
+        qq[s] = v / (double)M_;
+    }
+    return hipSuccess;
+}
+
+template class PropEngine<float>;
+template class PropEngine<double>;
+
+}  // namespace scft

@@ -1,0 +1,81 @@
+// prop_engine.hpp -- host-side driver of the batched propagator (one density::props per
+// batch slot, density.h:344-459), templated on the propagator's arithmetic type.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "prop_types.hpp"
+
+namespace scft {
+
+template <typename T>
+struct KernelSet {
+    int m0, m1, m2;
+    hipError_t (*setup)();
+    void (*begin)(const PropArgs<T>&, hipStream_t);
+    void (*step)(const PropArgs<T>&, int s, hipStream_t);
+};
+
+// kernel set for an (m0, m1, m2) grid, or nullptr when no instantiation exists
+template <typename T>
+const KernelSet<T>* find_kernels(int m0, int m1, int m2);
+
+// list of supported grids as text (for error messages)
+std::string supported_grids(bool fp64);
+
+template <typename T>
+class PropEngine {
+public:
+    PropEngine() = default;
+    PropEngine(const PropEngine&) = delete;
+    PropEngine& operator=(const PropEngine&) = delete;
+    ~PropEngine();
+
+    // Allocates buffers for nbmax slots. Returns "" on success, else an error message.
+    std::string init(const int m[3], const double D[3], int N, int nbmax);
+    // Recomputes the kinetic factor tables for a new box D.
+    hipError_t set_box(const double D[3]);
+
+    // props for nb slots: W row of slot i = dW + slot2rep[i]*wstride (device), phi likewise.
+    // Writes phis [nb][5] (device) and per-slot scalars (scal()).
+    hipError_t run(const double* dW, long long wstride, double* dphi, long long phistride,
+                   const int* d_slot2rep, int nb, const ChainConst& cc, double* d_phis,
+                   hipStream_t st);
+
+    // device [nbmax][8]: lnQ, sC, sH, QC, Q2, Q1, QC', Q2'
+    const double* scal() const { return scal_; }
+    // qq[s] = <q1'[s] q2'[N-s]> for s = 0..N of one slot (host array of N+1)
+    hipError_t qq_invariant(int slot, double* qq, hipStream_t st);
+    const cpx<T>* qstore() const { return qs_; }
+    long long M() const { return M_; }
+    int N() const { return N_; }
+    int nbmax() const { return nbmax_; }
+    size_t bytes_allocated() const { return bytes_; }
+    // HIP events around the contour loop of the next run() calls (diagnostics)
+    hipError_t enable_timing(bool on);
+    // elapsed ms of the contour loop of the last run() (after the stream is synchronised)
+    double loop_ms() const;
+
+private:
+    int m_[3] = {0, 0, 0};
+    double D_[3] = {0, 0, 0};
+    long long M_ = 0;
+    int N_ = 0, nbmax_ = 0, nblk_ = 0;
+    const KernelSet<T>* ks_ = nullptr;
+    cpx<T>* Ch_ = nullptr;
+    cpx<T>* Cf_ = nullptr;
+    T* e4A_ = nullptr;
+    T* e4B_ = nullptr;
+    cpx<T>* qs_ = nullptr;
+    cpx<T>* tw_[3] = {nullptr, nullptr, nullptr};
+    T* g_[3] = {nullptr, nullptr, nullptr};
+    double* part_ = nullptr;
+    double* cAB_ = nullptr;
+    double* scal_ = nullptr;
+    size_t bytes_ = 0;
+    bool timing_ = false;
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+};
+
+}  // namespace scft

@@ -1,0 +1,572 @@
+// scft_api.hip -- C-ABI implementation: context, batched props, and the host control flow
+// of solve_field0 / solve_field / FreeE with all field arithmetic on the device (FP64).
+#include "scft_api.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "field_kernels.hpp"
+#include "prop_engine.hpp"
+
+using scft::ChainConst;
+using scft::PropEngine;
+
+struct scft_ctx {
+    scft_params p{};
+    long long M = 0;
+    int R = 0;
+    double f_used = 0, ds = 0;
+    ChainConst cc{};
+    hipStream_t st = nullptr;
+    std::string err;
+    std::unique_ptr<PropEngine<float>> pe32;
+    std::unique_ptr<PropEngine<double>> pe64;
+    // field state
+    double *dW = nullptr, *dphi = nullptr, *dDEV = nullptr, *ddWda = nullptr;
+    // scratch rows for scft_props_batch
+    double *dWs = nullptr, *dphis_rows = nullptr;
+    // reductions / small arrays
+    double* dpart = nullptr;
+    int nblk = 0;
+    double *dsm = nullptr, *dlam = nullptr, *dx = nullptr, *dxref = nullptr, *dcproj = nullptr, *dphisv = nullptr;
+    int *dactive = nullptr, *dslot = nullptr;
+    // pinned host staging
+    double* hsm = nullptr;  // [R*8]
+    int* hslot = nullptr;   // [R]
+    double* hsmall = nullptr;  // [4R] x, xref, lam, cproj
+    int* hact = nullptr;    // [R]
+};
+
+namespace {
+
+scft_status fail(scft_ctx* c, scft_status s, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return s;
+}
+
+#define HIPCHK(ctx, expr)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail((ctx), SCFT_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+#define SCFTCHK(expr)                         \
+    do {                                      \
+        scft_status s_ = (expr);              \
+        if (s_ != SCFT_OK) return s_;         \
+    } while (0)
+
+// props for the slots listed in h_slots (replica indices into W/phi rows).
+scft_status run_props(scft_ctx* c, const double* dW, long long wstride, double* dphi, long long phistride,
+                      const std::vector<int>& slots, double* lnQ_host, double* phis_host)
+{
+    const int nb = (int)slots.size();
+    if (nb == 0) return SCFT_OK;
+    for (int i = 0; i < nb; ++i) c->hslot[i] = slots[i];
+    HIPCHK(c, hipMemcpyAsync(c->dslot, c->hslot, sizeof(int) * nb, hipMemcpyHostToDevice, c->st));
+    hipError_t e = c->pe32 ? c->pe32->run(dW, wstride, dphi, phistride, c->dslot, nb, c->cc, c->dphisv, c->st)
+                           : c->pe64->run(dW, wstride, dphi, phistride, c->dslot, nb, c->cc, c->dphisv, c->st);
+    if (e != hipSuccess) return fail(c, SCFT_ERR_HIP, std::string("propagator launch: ") + hipGetErrorString(e));
+    if (lnQ_host || phis_host) {
+        const double* scal = c->pe32 ? c->pe32->scal() : c->pe64->scal();
+        HIPCHK(c, hipMemcpyAsync(c->hsm, scal, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, c->st));
+        std::vector<double> ph(5 * (size_t)nb);
+        if (phis_host)
+            HIPCHK(c, hipMemcpyAsync(ph.data(), c->dphisv, sizeof(double) * 5 * nb, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        for (int i = 0; i < nb; ++i) {
+            if (lnQ_host) lnQ_host[i] = c->hsm[8 * i + 0];
+            if (phis_host)
+                for (int j = 0; j < 5; ++j) phis_host[5 * i + j] = ph[5 * i + j];
+        }
+    }
+    return SCFT_OK;
+}
+
+// finalize partial sums [nslots][nblk][ncomp] and bring them to host (hsm)
+scft_status finalize_to_host(scft_ctx* c, int nslots, int ncomp, std::vector<double>& out)
+{
+    out.assign((size_t)nslots * ncomp, 0.0);
+    if (nslots == 0) return SCFT_OK;
+    scft::launch_finalize(c->dpart, nslots, c->nblk, ncomp, c->dsm, c->st);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->hsm, c->dsm, sizeof(double) * nslots * ncomp, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    std::memcpy(out.data(), c->hsm, sizeof(double) * nslots * ncomp);
+    return SCFT_OK;
+}
+
+std::vector<int> fixed_doiis(const scft_ctx* c)
+{
+    std::vector<int> d(c->R, 1);  // single process owns every replica (scft.cu:952-975)
+    if (c->p.fix[0] == 1) d[0] = 0;  // scft.cu:587-588
+    if (c->p.fix[1] == 1) d[c->R - 1] = 0;
+    return d;
+}
+
+// squared neighbour distances over W- (scft.cu:351-355) -> host del2[R-1]
+scft_status edist2(scft_ctx* c, std::vector<double>& del2)
+{
+    del2.assign(c->R > 1 ? c->R - 1 : 0, 0.0);
+    if (c->R < 2) return SCFT_OK;
+    scft::launch_edist(c->R, c->M, c->dW, c->dpart, c->nblk, c->st);
+    HIPCHK(c, hipGetLastError());
+    return finalize_to_host(c, c->R - 1, 1, del2);
+}
+
+// scft.cu:612-621 / 725-736: normalised arc-length positions from W- distances
+void arc_positions(int R, const std::vector<double>& del2, std::vector<double>& x)
+{
+    x.assign(R, 0.0);
+    if (R < 2) return;
+    std::vector<double> Del(R - 1);
+    double Delsum = 0;
+    for (int i = 0; i < R - 1; ++i) {
+        Del[i] = std::sqrt(del2[i]);
+        Delsum += Del[i];
+    }
+    for (int i = 0; i < R - 1; ++i) Del[i] /= Delsum;
+    x[0] = 0;
+    x[R - 1] = 1;
+    for (int i = 1; i < R - 1; ++i) x[i] = x[i - 1] + Del[i - 1];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* scft_last_error(const scft_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
+{
+    if (!p || !out) return SCFT_ERR_INVALID;
+    *out = nullptr;
+    auto c = std::make_unique<scft_ctx>();
+    c->p = *p;
+    c->R = p->nrep;
+    c->M = (long long)p->m[0] * p->m[1] * p->m[2];
+    if (p->m[0] <= 0 || p->m[1] <= 0 || p->m[2] <= 0 || p->N <= 1 || p->nrep <= 0)
+        return SCFT_ERR_INVALID;
+    if (p->ens != 1 && p->ens != 2) return SCFT_ERR_INVALID;
+    if (p->precision != SCFT_FP64 && p->precision != SCFT_FP32) return SCFT_ERR_INVALID;
+    if (p->nrep > scft::SPLINE_RMAX) return SCFT_ERR_UNSUPPORTED;
+    // f <- round(N f)/N (scft.cu:1008); props: n1 = round(Ns f), n2 = Ns - n1, Nh = (int)(alpha Ns)
+    c->f_used = (double)std::round(p->N * p->f) / p->N;
+    c->ds = 1.0 / p->N;
+    c->cc.N = p->N;
+    c->cc.n1 = (int)std::round(p->N * c->f_used);
+    c->cc.n2 = p->N - c->cc.n1;
+    c->cc.Nh = (int)(p->alpha * p->N);
+    c->cc.ens = p->ens;
+    c->cc.alpha = p->alpha;
+    c->cc.phic = p->phidb;
+    c->cc.ds = c->ds;
+    if (c->cc.n1 <= 0 || c->cc.n1 >= p->N || c->cc.Nh < 0 || c->cc.Nh > c->cc.n2) return SCFT_ERR_INVALID;
+
+    if (hipSetDevice(p->device) != hipSuccess) return SCFT_ERR_HIP;
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return SCFT_ERR_HIP;
+    std::string msg;
+    if (p->precision == SCFT_FP32) {
+        c->pe32 = std::make_unique<PropEngine<float>>();
+        msg = c->pe32->init(p->m, p->D, p->N, c->R);
+    } else {
+        c->pe64 = std::make_unique<PropEngine<double>>();
+        msg = c->pe64->init(p->m, p->D, p->N, c->R);
+    }
+    if (!msg.empty()) {
+        fprintf(stderr, "scft_ctx_create: %s\n", msg.c_str());
+        (void)hipStreamDestroy(c->st);
+        c->st = nullptr;
+        return msg.find("no ") == 0 ? SCFT_ERR_UNSUPPORTED : SCFT_ERR_NOMEM;
+    }
+    const long long M = c->M;
+    const int R = c->R;
+    c->nblk = (int)std::min<long long>(1024, (M + scft::FK_NT - 1) / scft::FK_NT);
+    bool ok = hipMalloc((void**)&c->dW, sizeof(double) * 2 * M * R) == hipSuccess &&
+              hipMalloc((void**)&c->dphi, sizeof(double) * 4 * M * R) == hipSuccess &&
+              hipMalloc((void**)&c->dDEV, sizeof(double) * 2 * M * R) == hipSuccess &&
+              hipMalloc((void**)&c->ddWda, sizeof(double) * M * R) == hipSuccess &&
+              hipMalloc((void**)&c->dWs, sizeof(double) * 2 * M * R) == hipSuccess &&
+              hipMalloc((void**)&c->dphis_rows, sizeof(double) * 4 * M * R) == hipSuccess &&
+              hipMalloc((void**)&c->dpart, sizeof(double) * (size_t)c->nblk * 4 * R) == hipSuccess &&
+              hipMalloc((void**)&c->dsm, sizeof(double) * 8 * R) == hipSuccess &&
+              hipMalloc((void**)&c->dlam, sizeof(double) * R) == hipSuccess &&
+              hipMalloc((void**)&c->dx, sizeof(double) * R) == hipSuccess &&
+              hipMalloc((void**)&c->dxref, sizeof(double) * R) == hipSuccess &&
+              hipMalloc((void**)&c->dcproj, sizeof(double) * R) == hipSuccess &&
+              hipMalloc((void**)&c->dphisv, sizeof(double) * 5 * R) == hipSuccess &&
+              hipMalloc((void**)&c->dactive, sizeof(int) * R) == hipSuccess &&
+              hipMalloc((void**)&c->dslot, sizeof(int) * R) == hipSuccess &&
+              hipHostMalloc((void**)&c->hsm, sizeof(double) * 8 * R, hipHostMallocDefault) == hipSuccess &&
+              hipHostMalloc((void**)&c->hslot, sizeof(int) * R, hipHostMallocDefault) == hipSuccess &&
+              hipHostMalloc((void**)&c->hsmall, sizeof(double) * 4 * R, hipHostMallocDefault) == hipSuccess &&
+              hipHostMalloc((void**)&c->hact, sizeof(int) * R, hipHostMallocDefault) == hipSuccess;
+    if (!ok) {
+        scft_ctx_destroy(c.release());
+        return SCFT_ERR_NOMEM;
+    }
+    if (hipMemsetAsync(c->dW, 0, sizeof(double) * 2 * M * R, c->st) != hipSuccess ||
+        hipMemsetAsync(c->dphi, 0, sizeof(double) * 4 * M * R, c->st) != hipSuccess ||
+        hipMemsetAsync(c->ddWda, 0, sizeof(double) * M * R, c->st) != hipSuccess ||
+        hipStreamSynchronize(c->st) != hipSuccess) {
+        scft_ctx_destroy(c.release());
+        return SCFT_ERR_HIP;
+    }
+    *out = c.release();
+    return SCFT_OK;
+}
+
+scft_status scft_ctx_destroy(scft_ctx* c)
+{
+    if (!c) return SCFT_OK;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    c->pe32.reset();
+    c->pe64.reset();
+    void* dptrs[] = {c->dW, c->dphi, c->dDEV, c->ddWda, c->dWs, c->dphis_rows, c->dpart, c->dsm, c->dlam,
+                     c->dx, c->dxref, c->dcproj, c->dphisv, c->dactive, c->dslot};
+    for (void* q : dptrs)
+        if (q) (void)hipFree(q);
+    void* hptrs[] = {c->hsm, c->hslot, c->hsmall, c->hact};
+    for (void* q : hptrs)
+        if (q) (void)hipHostFree(q);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return SCFT_OK;
+}
+
+scft_status scft_sync(scft_ctx* c)
+{
+    if (!c) return SCFT_ERR_INVALID;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return SCFT_OK;
+}
+
+scft_status scft_props_batch(scft_ctx* c, const double* W, double* phi, double* lnQ, double* phis, int nrep)
+{
+    if (!c || !W || nrep <= 0 || nrep > c->R) return fail(c, SCFT_ERR_INVALID, "scft_props_batch: bad arguments");
+    const long long M = c->M;
+    HIPCHK(c, hipMemcpyAsync(c->dWs, W, sizeof(double) * 2 * M * nrep, hipMemcpyHostToDevice, c->st));
+    std::vector<int> slots(nrep);
+    for (int i = 0; i < nrep; ++i) slots[i] = i;
+    std::vector<double> lq(nrep), ph(5 * (size_t)nrep);
+    SCFTCHK(run_props(c, c->dWs, 2 * M, c->dphis_rows, 4 * M, slots, lq.data(), ph.data()));
+    if (phi) HIPCHK(c, hipMemcpyAsync(phi, c->dphis_rows, sizeof(double) * 4 * M * nrep, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (lnQ) std::memcpy(lnQ, lq.data(), sizeof(double) * nrep);
+    if (phis) std::memcpy(phis, ph.data(), sizeof(double) * 5 * nrep);
+    return SCFT_OK;
+}
+
+scft_status scft_props_batch_device(scft_ctx* c, const double* dW, double* dphi, double* lnQ, double* phis,
+                                    int nrep)
+{
+    if (!c || !dW || !dphi || nrep <= 0 || nrep > c->R)
+        return fail(c, SCFT_ERR_INVALID, "scft_props_batch_device: bad arguments");
+    std::vector<int> slots(nrep);
+    for (int i = 0; i < nrep; ++i) slots[i] = i;
+    std::vector<double> lq(nrep), ph(5 * (size_t)nrep);
+    SCFTCHK(run_props(c, dW, 2 * c->M, dphi, 4 * c->M, slots, lq.data(), ph.data()));
+    if (lnQ) std::memcpy(lnQ, lq.data(), sizeof(double) * nrep);
+    if (phis) std::memcpy(phis, ph.data(), sizeof(double) * 5 * nrep);
+    return SCFT_OK;
+}
+
+scft_status scft_set_fields(scft_ctx* c, const double* W)
+{
+    if (!c || !W) return fail(c, SCFT_ERR_INVALID, "scft_set_fields: null");
+    HIPCHK(c, hipMemcpyAsync(c->dW, W, sizeof(double) * 2 * c->M * c->R, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return SCFT_OK;
+}
+
+scft_status scft_get_fields(scft_ctx* c, double* W)
+{
+    if (!c || !W) return fail(c, SCFT_ERR_INVALID, "scft_get_fields: null");
+    HIPCHK(c, hipMemcpyAsync(W, c->dW, sizeof(double) * 2 * c->M * c->R, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return SCFT_OK;
+}
+
+scft_status scft_get_densities(scft_ctx* c, double* phi)
+{
+    if (!c || !phi) return fail(c, SCFT_ERR_INVALID, "scft_get_densities: null");
+    HIPCHK(c, hipMemcpyAsync(phi, c->dphi, sizeof(double) * 4 * c->M * c->R, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return SCFT_OK;
+}
+
+scft_status scft_get_tangent(scft_ctx* c, double* dWda)
+{
+    if (!c || !dWda) return fail(c, SCFT_ERR_INVALID, "scft_get_tangent: null");
+    HIPCHK(c, hipMemcpyAsync(dWda, c->ddWda, sizeof(double) * c->M * c->R, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return SCFT_OK;
+}
+
+// solve_field0 (scft.cu:516-554), every flagged replica in lockstep with its own lambda and exit
+scft_status scft_solve_w_plus(scft_ctx* c, int maxIter, double errTol, const int* which, int* k_out)
+{
+    if (!c) return SCFT_ERR_INVALID;
+    const int R = c->R;
+    const long long M = c->M;
+    std::vector<int> act(R), k(R, 1);
+    std::vector<double> err(R, 1.0), errt(R, 1.0), lam(R, 0.05);
+    for (int ii = 0; ii < R; ++ii) act[ii] = which ? (which[ii] != 0) : 1;
+    const double dlam1 = 1.09, dlam2 = 1.10;
+    std::vector<double> s1;
+    for (;;) {
+        std::vector<int> slots;
+        for (int ii = 0; ii < R; ++ii)
+            if (act[ii] && k[ii] < maxIter && err[ii] > errTol) slots.push_back(ii);
+        if (slots.empty()) break;
+        const int nb = (int)slots.size();
+        for (int ii = 0; ii < R; ++ii) c->hsmall[ii] = lam[ii];
+        HIPCHK(c, hipMemcpyAsync(c->dlam, c->hsmall, sizeof(double) * R, hipMemcpyHostToDevice, c->st));
+        SCFTCHK(run_props(c, c->dW, 2 * M, c->dphi, 4 * M, slots, nullptr, nullptr));
+        scft::launch_wplus_mix(c->dslot, nb, M, c->p.chi, c->dlam, c->dphi, c->dW, c->dDEV, c->dpart, c->nblk,
+                               c->st);
+        HIPCHK(c, hipGetLastError());
+        SCFTCHK(finalize_to_host(c, nb, 1, s1));
+        for (int j = 0; j < nb; ++j) {
+            const int ii = slots[j];
+            err[ii] = std::pow(s1[j] / (M), 0.5);
+            if (!(err[ii] == err[ii])) {
+                char b[160];
+                snprintf(b, sizeof b, "solve_field0: W+ error is NaN for replica %d at k=%d", ii, k[ii]);
+                return fail(c, SCFT_ERR_NAN, b);
+            }
+            if (err[ii] < errt[ii]) lam[ii] = std::min(0.1, lam[ii] * dlam1);
+            else lam[ii] = std::max(0.005, lam[ii] / dlam2);
+            errt[ii] = err[ii];
+            k[ii]++;
+        }
+    }
+    if (k_out)
+        for (int ii = 0; ii < R; ++ii) k_out[ii] = act[ii] ? k[ii] : 0;
+    return SCFT_OK;
+}
+
+// one solve_field iteration (scft.cu:592-757), single process, doii = -1
+scft_status scft_string_iterate(scft_ctx* c, int k, double errTol, double* alf, scft_iter_report* rep)
+{
+    if (!c) return SCFT_ERR_INVALID;
+    const int R = c->R;
+    const long long M = c->M;
+    const double lambda = 0.05, eps = 0.5;  // scft.cu:562, 570 (lambda stays 0.05: A.1)
+    int dospring = 1, doredist = 0, doperp = 0;
+    if (c->p.dostring != 1) dospring = 0;
+    if (k >= c->p.springsteps && c->p.dostring == 1) {
+        dospring = 0;
+        doredist = 1;
+        doperp = 1;
+    }
+    const std::vector<int> doiis = fixed_doiis(c);
+    std::vector<int> active;
+    for (int ii = 0; ii < R; ++ii)
+        if (doiis[ii]) active.push_back(ii);
+
+    HIPCHK(c, hipMemsetAsync(c->dDEV, 0, sizeof(double) * 2 * M * R, c->st));
+    std::vector<int> kk(R, 0), steps(R, 0);
+    SCFTCHK(scft_solve_w_plus(c, 20, errTol, doiis.data(), kk.data()));
+    for (int ii = 0; ii < R; ++ii) steps[ii] = doiis[ii] ? 1 + kk[ii] : 0;
+    SCFTCHK(run_props(c, c->dW, 2 * M, c->dphi, 4 * M, active, nullptr, nullptr));
+    const int nb = (int)active.size();
+    scft::launch_dev_full(c->dslot, nb, R, M, c->p.chi, dospring, eps, c->dphi, c->dW, c->dDEV, c->dpart, c->nblk,
+                          c->st);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> devs;  // [slot][2]: sum DEV^2 (2M), sum DEV-^2 (M)
+    SCFTCHK(finalize_to_host(c, nb, 2, devs));
+    std::vector<double> del2, x, xref(R);
+    SCFTCHK(edist2(c, del2));
+    arc_positions(R, del2, x);
+    for (int ii = 0; ii < R; ++ii) xref[ii] = (1.0 * ii) / (R - 1.0);
+
+    std::vector<double> errs(R, 0.0), errs3(R, 0.0), normDER(R, 0.0), normDEV(R, 0.0), dotWs(R, 0.0),
+        dotWs2(R, 0.0);
+    std::vector<double> sdev(R, 0.0), sdevm(R, 0.0);
+    for (int j = 0; j < nb; ++j) {
+        sdev[active[j]] = devs[2 * j];
+        sdevm[active[j]] = devs[2 * j + 1];
+    }
+    for (int ii = 0; ii < R; ++ii) errs[ii] = std::pow(sdev[ii] / (2 * M), 0.5);
+
+    std::vector<int> proj(R, 0);  // interior replicas whose DEV- is projected (scft.cu:670-678)
+    for (int ii = 0; ii < R; ++ii) c->hsmall[3 * R + ii] = 0.0;  // cproj
+    if (doperp == 1) {
+        for (int ii = 0; ii < R; ++ii) {
+            c->hsmall[ii] = x[ii];
+            c->hact[ii] = doiis[ii];
+        }
+        HIPCHK(c, hipMemcpyAsync(c->dx, c->hsmall, sizeof(double) * R, hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(c->dactive, c->hact, sizeof(int) * R, hipMemcpyHostToDevice, c->st));
+        scft::launch_spline_deriv(R, M, c->dx, c->dactive, c->dW, c->ddWda, c->st);
+        HIPCHK(c, hipGetLastError());
+        scft::launch_tangent_dots(c->dslot, nb, M, c->ddWda, c->dDEV, c->dpart, c->nblk, c->st);
+        HIPCHK(c, hipGetLastError());
+        std::vector<double> td;
+        SCFTCHK(finalize_to_host(c, nb, 2, td));
+        for (int j = 0; j < nb; ++j) {
+            const int ii = active[j];
+            normDER[ii] = std::sqrt(td[2 * j]);
+            normDEV[ii] = std::sqrt(sdevm[ii]);
+            dotWs[ii] = td[2 * j + 1] / (normDEV[ii] * normDER[ii]);
+        }
+        for (int ii = 1; ii < R - 1; ++ii) {
+            if (doiis[ii] == 1 && normDER[ii] > 0 && normDEV[ii] > 0) {
+                proj[ii] = 1;
+                c->hsmall[3 * R + ii] = (dotWs[ii] / normDER[ii]) * normDEV[ii];
+            } else if (doiis[ii] == 1) {
+                fprintf(stderr, "Fatal error occurred (0)... %d %d %lf %lf\n", k, ii, normDER[ii], normDEV[ii]);
+            }
+        }
+    }
+    HIPCHK(c, hipMemcpyAsync(c->dcproj, c->hsmall + 3 * R, sizeof(double) * R, hipMemcpyHostToDevice, c->st));
+    scft::launch_project_mix(R, M, c->dcproj, lambda, c->ddWda, c->dDEV, c->dW, c->dpart, c->nblk, c->st);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> pm;
+    SCFTCHK(finalize_to_host(c, R, 2, pm));
+    if (doperp == 1) {
+        for (int ii = 0; ii < R; ++ii) {
+            errs3[ii] = proj[ii] ? std::pow(pm[2 * ii] / (2 * M), 0.5) : 0.0;
+            // DEV- of a replica that was not projected (or projected by 0) is unchanged
+            if (doiis[ii] == 1)
+                dotWs2[ii] = (c->hsmall[3 * R + ii] != 0.0) ? pm[2 * ii + 1] / (normDEV[ii] * normDER[ii])
+                                                             : dotWs[ii];
+        }
+    } else {
+        for (int ii = 0; ii < R; ++ii) errs3[ii] = errs[ii];
+    }
+    errs3[0] = errs[0];
+    errs3[R - 1] = errs[R - 1];
+
+    // arc length of the mixed string (scft.cu:725-743)
+    SCFTCHK(edist2(c, del2));
+    arc_positions(R, del2, x);
+    double fitup = 0;
+    for (int ii = 1; ii < R - 1; ++ii) fitup += std::pow(x[ii] - xref[ii], 2.0);
+    fitup /= R - 2.0;
+    fitup = std::sqrt(fitup);
+    fitup /= R - 1.0;
+    if (alf)
+        for (int ii = 0; ii < R; ++ii) alf[ii] = x[ii];
+
+    if (doredist == 1) {
+        for (int ii = 0; ii < R; ++ii) {
+            c->hsmall[ii] = x[ii];
+            c->hsmall[R + ii] = xref[ii];
+            c->hact[ii] = doiis[ii];
+        }
+        HIPCHK(c, hipMemcpyAsync(c->dx, c->hsmall, sizeof(double) * R, hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(c->dxref, c->hsmall + R, sizeof(double) * R, hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(c->dactive, c->hact, sizeof(int) * R, hipMemcpyHostToDevice, c->st));
+        scft::launch_redistribute(R, M, c->dx, c->dxref, c->dactive, lambda, c->dW, c->st);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+
+    if (rep) {
+        double stepav = 0;
+        for (int ii = 0; ii < R; ++ii) stepav += steps[ii];
+        stepav /= R;
+        auto put = [R](double* dst, const std::vector<double>& v) {
+            if (dst) std::memcpy(dst, v.data(), sizeof(double) * R);
+        };
+        put(rep->errs, errs);
+        put(rep->errs3, errs3);
+        put(rep->x, x);
+        put(rep->xref, xref);
+        put(rep->dotWs, dotWs);
+        put(rep->dotWs2, dotWs2);
+        put(rep->normDER, normDER);
+        put(rep->normDEV, normDEV);
+        if (rep->steps) std::memcpy(rep->steps, steps.data(), sizeof(int) * R);
+        rep->fitup = fitup;
+        rep->stepav = stepav;
+    }
+    return SCFT_OK;
+}
+
+// FreeE (scft.cu:812-853), single process
+scft_status scft_free_energy(scft_ctx* c, double* FE, double* FEs, double* FEs0, double* FEs1, double* phctot,
+                             double* alf)
+{
+    if (!c) return SCFT_ERR_INVALID;
+    const int R = c->R;
+    const long long M = c->M;
+    const std::vector<int> doiis = fixed_doiis(c);
+    std::vector<int> which(R, 0), set;
+    for (int ii = 0; ii < R; ++ii)
+        if (doiis[ii] == 1 || ii == 0 || ii == R - 1) {
+            which[ii] = 1;
+            set.push_back(ii);
+        }
+    SCFTCHK(scft_solve_w_plus(c, 100, 1e-4, which.data(), nullptr));
+    const int nb = (int)set.size();
+    std::vector<double> lq(nb);
+    SCFTCHK(run_props(c, c->dW, 2 * M, c->dphi, 4 * M, set, lq.data(), nullptr));
+    scft::launch_fe_sums(c->dslot, nb, M, c->p.chi, c->dW, c->dphi, c->dpart, c->nblk, c->st);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> fs;
+    SCFTCHK(finalize_to_host(c, nb, 2, fs));
+    std::vector<double> vFEs(R, 0.0), vFEs0(R, 0.0), vFEs1(R, 0.0), vph(R, 0.0), valf(R, 0.0);
+    for (int j = 0; j < nb; ++j) {
+        const int ii = set[j];
+        vFEs0[ii] = -lq[j];
+        vFEs1[ii] = fs[2 * j] / M;
+        vph[ii] = fs[2 * j + 1] / (M * c->f_used);
+        vFEs[ii] = vFEs0[ii] + vFEs1[ii];
+    }
+    std::vector<double> del2;
+    SCFTCHK(edist2(c, del2));
+    valf[0] = 0;
+    for (int ii = 1; ii < R; ++ii) valf[ii] = valf[ii - 1] + std::sqrt(del2[ii - 1]);
+    double tot = 0;
+    for (int ii = 0; ii < R; ++ii) tot += vFEs[ii];
+    auto put = [R](double* dst, const std::vector<double>& v) {
+        if (dst) std::memcpy(dst, v.data(), sizeof(double) * R);
+    };
+    put(FEs, vFEs);
+    put(FEs0, vFEs0);
+    put(FEs1, vFEs1);
+    put(phctot, vph);
+    put(alf, valf);
+    if (FE) *FE = tot;
+    return SCFT_OK;
+}
+
+scft_status scft_debug_qq_invariant(scft_ctx* c, int slot, double* qq)
+{
+    if (!c || !qq || slot < 0 || slot >= c->R) return SCFT_ERR_INVALID;
+    hipError_t e = c->pe32 ? c->pe32->qq_invariant(slot, qq, c->st) : c->pe64->qq_invariant(slot, qq, c->st);
+    if (e != hipSuccess) return fail(c, SCFT_ERR_HIP, hipGetErrorString(e));
+    return SCFT_OK;
+}
+
+scft_status scft_debug_time_props(scft_ctx* c, int nrep, int reps, double* ms_per_step)
+{
+    if (!c || !ms_per_step || nrep <= 0 || nrep > c->R || reps <= 0) return SCFT_ERR_INVALID;
+    std::vector<int> slots(nrep);
+    for (int i = 0; i < nrep; ++i) slots[i] = i;
+    hipError_t e = c->pe32 ? c->pe32->enable_timing(true) : c->pe64->enable_timing(true);
+    if (e != hipSuccess) return fail(c, SCFT_ERR_HIP, hipGetErrorString(e));
+    double tot = 0;
+    for (int r = 0; r < reps; ++r) {
+        SCFTCHK(run_props(c, c->dW, 2 * c->M, c->dphi, 4 * c->M, slots, nullptr, nullptr));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        tot += c->pe32 ? c->pe32->loop_ms() : c->pe64->loop_ms();
+    }
+    if (c->pe32) (void)c->pe32->enable_timing(false);
+    else (void)c->pe64->enable_timing(false);
+    *ms_per_step = tot / reps / c->p.N;
+    return SCFT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,169 @@
+"""HIP path vs the FP64 oracle, through the C-ABI (libscft_hip.so).
+
+FP64 bar: 1e-10 relative on Q (lnQ), the densities, the RMS exchange-chemical-potential
+errors and the free energies. FP32 results carry their measured error against the oracle.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import orc
+from scft_string_amd import Context
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _fields(m, D, seed, amp=2.0, mean_plus=0.0):
+    rng = np.random.default_rng(seed)
+    M = int(np.prod(m))
+    x = (np.arange(m[0]) * D[0] / m[0])[:, None, None] * np.ones(m)
+    Wm = amp * np.cos(2 * np.pi * x / D[0]).reshape(-1) + 0.3 * rng.standard_normal(M)
+    Wp = mean_plus + 0.2 * rng.standard_normal(M)
+    return np.stack([Wm, Wp])
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+CASES = [
+    # m, D, N, f, alpha, ens, phidb
+    ((8, 8, 8), (2.0, 2.0, 2.0), 20, 0.5, 0.1, 1, 0.8),
+    ((12, 8, 16), (3.0, 2.5, 4.0), 30, 0.4, 0.2, 1, 0.7),
+    ((16, 16, 16), (4.0, 4.0, 4.0), 40, 0.45, 0.1, 2, 0.9),
+    ((8, 12, 10), (2.0, 3.0, 2.5), 25, 0.52, 0.12, 2, 0.3),
+    ((32, 32, 32), (4.0, 4.0, 4.0), 50, 0.5, 0.1, 1, 0.8),
+    ((24, 24, 24), (3.0, 3.0, 3.0), 33, 0.39, 0.1, 1, 0.6),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "x".join(map(str, c[0])) + f"_N{c[2]}_ens{c[5]}")
+def test_props_fp64_matches_oracle(case):
+    m, D, N, f, alpha, ens, phidb = case
+    _say("props fp64", case)
+    W = np.stack([_fields(m, D, seed=11 + i, mean_plus=0.4 * i) for i in range(3)])
+    g = orc.Grid(m, D)
+    with Context(m, D, N, f, alpha, ens, phidb, chi=15.0, nrep=3, precision="fp64") as ctx:
+        phi, lnQ, phis = ctx.props_batch(W)
+    for i in range(3):
+        ref = g.props(W[i], N=N, alpha=alpha, phic=phidb, f=f, ens=ens)
+        assert abs(lnQ[i] - ref["lnQ"]) <= TOL64 * max(1.0, abs(ref["lnQ"])), (i, lnQ[i], ref["lnQ"])
+        for c in range(4):
+            assert _rel(phi[i, c], ref["phi"][c]) <= TOL64, (i, c, _rel(phi[i, c], ref["phi"][c]))
+        np.testing.assert_allclose(phis[i], ref["phis"], rtol=TOL64, atol=1e-14)
+
+
+@pytest.mark.parametrize("m", [(16, 16, 16), (32, 32, 32)])
+def test_props_fp32_error_vs_oracle(m):
+    D = (4.0, 4.0, 4.0)
+    N, f, alpha, ens, phidb = 40, 0.45, 0.1, 1, 0.85
+    W = _fields(m, D, seed=3)[None]
+    g = orc.Grid(m, D)
+    ref = g.props(W[0], N=N, alpha=alpha, phic=phidb, f=f, ens=ens)
+    with Context(m, D, N, f, alpha, ens, phidb, chi=15.0, nrep=1, precision="fp32") as ctx:
+        phi, lnQ, phis = ctx.props_batch(W)
+    dl = abs(lnQ[0] - ref["lnQ"])
+    dphi = max(_rel(phi[0, c], ref["phi"][c]) for c in range(4))
+    _say(f"fp32 {m}: |dlnQ|={dl:.3e} max rel dphi={dphi:.3e}")
+    assert dl < 1e-5
+    assert dphi < 1e-4
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+def test_symmetric_operator_invariant(prec, tol):
+    """<q1[s] q2[N-s]> is independent of s (fails for a wrong species switch, factor
+    order or Richardson weight)."""
+    m, D = (32, 32, 32), (4.0, 4.0, 4.0)
+    N = 48
+    W = _fields(m, D, seed=5, amp=3.0)[None]
+    with Context(m, D, N, 0.4, 0.1, 1, 0.8, chi=15.0, nrep=1, precision=prec) as ctx:
+        ctx.props_batch(W)
+        qq = ctx.qq_invariant(0)
+    spread = (qq.max() - qq.min()) / qq.mean()
+    _say(f"{prec}: qq spread {spread:.3e}")
+    assert spread < tol
+
+
+def test_solve_w_plus_matches_oracle():
+    m, D = (12, 8, 16), (3.0, 2.5, 4.0)
+    N, f, alpha, ens, phidb, chi = 30, 0.4, 0.1, 2, 0.5, 12.0
+    R = 2
+    W = np.stack([_fields(m, D, seed=21 + i) for i in range(R)])
+    g = orc.Grid(m, D)
+    M = g.M
+    with Context(m, D, N, f, alpha, ens, phidb, chi=chi, nrep=R, precision="fp64") as ctx:
+        ctx.set_fields(W)
+        k = ctx.solve_w_plus(8, 1e-6)
+        Wg = ctx.get_fields()
+    for i in range(R):
+        row = np.zeros(7 * M)
+        row[:2 * M] = W[i].reshape(-1)
+        dev = np.zeros(2 * M)
+        kr = g.solve_field0(row, dev, chi, f, N, alpha, phidb, ens, 8, 1e-6)
+        assert k[i] == kr
+        assert _rel(Wg[i], row[:2 * M]) <= TOL64
+
+
+def _string_setup(R, m, D, seed):
+    M = int(np.prod(m))
+    rng = np.random.default_rng(seed)
+    x = (np.arange(m[0]) * D[0] / m[0])[:, None, None] * np.ones(m)
+    base = np.cos(2 * np.pi * x / D[0]).reshape(-1)
+    W = np.zeros((R, 2, M))
+    for i in range(R):
+        W[i, 0] = (0.6 + 0.5 * i) * base + 0.05 * rng.standard_normal(M)
+        W[i, 1] = 0.05 * rng.standard_normal(M)
+    return W
+
+
+@pytest.mark.parametrize("springsteps", [0, 3])
+def test_string_iteration_matches_oracle(springsteps):
+    m, D = (8, 12, 10), (2.0, 3.0, 2.5)
+    N, f, alpha, ens, phidb, chi = 24, 0.5, 0.125, 1, 0.8, 14.0
+    R = 5
+    W0 = _string_setup(R, m, D, seed=9)
+    g = orc.Grid(m, D)
+    run = dict(chi=chi, f=f, alpha=alpha, phidb=phidb, N=N, ens=ens, dostring=1, fix=(1, 1),
+               springsteps=springsteps)
+    st = orc.StringState(g, run, W0)
+    with Context(m, D, N, f, alpha, ens, phidb, chi=chi, nrep=R, precision="fp64", fix=(1, 1),
+                 springsteps=springsteps) as ctx:
+        ctx.set_fields(W0)
+        for k in range(2):
+            _say("string iteration", k, "springsteps", springsteps)
+            rep_g = ctx.string_iterate(k)
+            rep_o = st.iterate(k)
+            Wg = ctx.get_fields()
+            assert _rel(Wg, st.W[:, :2 * g.M]) <= TOL64, _rel(Wg, st.W[:, :2 * g.M])
+            np.testing.assert_array_equal(rep_g["steps"], rep_o["steps"])
+            for key in ("errs", "errs3", "x", "dotWs", "dotWs2"):
+                np.testing.assert_allclose(rep_g[key], rep_o[key], rtol=TOL64, atol=1e-13, err_msg=key)
+            # teacher forcing: both continue from the GPU state
+            st.W[:, :2 * g.M] = Wg
+
+
+def test_free_energy_matches_oracle():
+    m, D = (12, 8, 16), (3.0, 2.5, 4.0)
+    N, f, alpha, ens, phidb, chi = 30, 0.5, 0.1, 1, 0.8, 14.0
+    R = 4
+    W0 = _string_setup(R, m, D, seed=4)
+    g = orc.Grid(m, D)
+    run = dict(chi=chi, f=f, alpha=alpha, phidb=phidb, N=N, ens=ens, dostring=1, fix=(1, 1))
+    st = orc.StringState(g, run, W0)
+    fe_o = st.free_energy()
+    with Context(m, D, N, f, alpha, ens, phidb, chi=chi, nrep=R, precision="fp64", fix=(1, 1)) as ctx:
+        ctx.set_fields(W0)
+        fe_g = ctx.free_energy()
+        Wg = ctx.get_fields()
+    assert _rel(Wg, st.W[:, :2 * g.M]) <= TOL64
+    for key in ("FEs", "FEs0", "FEs1", "phctot", "alf"):
+        np.testing.assert_allclose(fe_g[key], fe_o[key], rtol=TOL64, atol=1e-12, err_msg=key)
+    assert abs(fe_g["FE"] - fe_o["FE"]) <= TOL64 * abs(fe_o["FE"])
