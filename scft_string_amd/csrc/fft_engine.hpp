@@ -22,25 +22,25 @@ struct alignas(2 * sizeof(T)) cpx {
 };
 
 template <typename T>
-__device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+__host__ __device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T>
-__device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+__host__ __device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
 template <typename T>
-__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b)
+__host__ __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b)
 {
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 // a * conj(b)
 template <typename T>
-__device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b)
+__host__ __device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b)
 {
     return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
 }
 template <typename T>
-__device__ __forceinline__ cpx<T> cscale(cpx<T> a, T s) { return {a.x * s, a.y * s}; }
+__host__ __device__ __forceinline__ cpx<T> cscale(cpx<T> a, T s) { return {a.x * s, a.y * s}; }
 // multiply by -i (forward) or +i (inverse)
 template <bool INV, typename T>
-__device__ __forceinline__ cpx<T> mul_mi(cpx<T> a)
+__host__ __device__ __forceinline__ cpx<T> mul_mi(cpx<T> a)
 {
     if (INV) return {-a.y, a.x};
     return {a.y, -a.x};
@@ -52,7 +52,7 @@ struct dft;
 
 template <bool INV, typename T>
 struct dft<2, INV, T> {
-    static __device__ __forceinline__ void run(cpx<T>* v)
+    static __host__ __device__ __forceinline__ void run(cpx<T>* v)
     {
         cpx<T> a = v[0], b = v[1];
         v[0] = cadd(a, b);
@@ -62,7 +62,7 @@ struct dft<2, INV, T> {
 
 template <bool INV, typename T>
 struct dft<3, INV, T> {
-    static __device__ __forceinline__ void run(cpx<T>* v)
+    static __host__ __device__ __forceinline__ void run(cpx<T>* v)
     {
         const T c = T(0.86602540378443864676372317075294);  // sqrt(3)/2
         cpx<T> s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
@@ -76,7 +76,7 @@ struct dft<3, INV, T> {
 
 template <bool INV, typename T>
 struct dft<4, INV, T> {
-    static __device__ __forceinline__ void run(cpx<T>* v)
+    static __host__ __device__ __forceinline__ void run(cpx<T>* v)
     {
         cpx<T> a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
         cpx<T> c = cadd(v[1], v[3]), d = mul_mi<INV>(csub(v[1], v[3]));
@@ -89,7 +89,7 @@ struct dft<4, INV, T> {
 
 template <bool INV, typename T>
 struct dft<5, INV, T> {
-    static __device__ __forceinline__ void run(cpx<T>* v)
+    static __host__ __device__ __forceinline__ void run(cpx<T>* v)
     {
         const T c1 = T(0.30901699437494742410229341718282);   // cos(2pi/5)
         const T c2 = T(-0.80901699437494742410229341718282);  // cos(4pi/5)
@@ -113,7 +113,7 @@ struct dft<5, INV, T> {
 
 template <bool INV, typename T>
 struct dft<8, INV, T> {
-    static __device__ __forceinline__ void run(cpx<T>* v)
+    static __host__ __device__ __forceinline__ void run(cpx<T>* v)
     {
         const T h = T(0.70710678118654752440084436210485);
         cpx<T> a[4], b[4];

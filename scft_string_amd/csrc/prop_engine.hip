@@ -1,14 +1,19 @@
 // prop_engine.hip -- kernel instantiation table and the batched props driver.
 #include "prop_engine.hpp"
 #include "prop_kernels.hpp"
+#include "prop_kernels2.hpp"
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 namespace scft {
 
 // ---------------------------------------------------------------- per-grid kernel sets
+// Two implementations per grid: the two-stage register FFT (prop_kernels2.hpp) when every
+// axis splits as N1 x N2 (1 < N1 <= N2 <= 16) and fits in LDS, and the LDS Stockham
+// kernels (prop_kernels.hpp) otherwise or when SCFT_LEGACY_FFT=1 is set.
 template <typename T, int N0, int N1, int N2>
 struct KS {
     static constexpr int NE = N1 * N2;
@@ -18,38 +23,96 @@ struct KS {
     static constexpr size_t lds_plane = (size_t)(N1 * PlaneGeom<N1, N2>::P2 + N1 + N2) * sizeof(cpx<T>);
     static constexpr size_t lds_col = (size_t)(N0 * TC + N0) * sizeof(cpx<T>) + (size_t)N0 * sizeof(T);
     static constexpr int ntile = (N1 * N2 + TC - 1) / TC;
-    static_assert(lds_plane <= 160 * 1024, "plane does not fit in LDS");
-    static_assert(lds_col <= 160 * 1024, "column tile does not fit in LDS");
+    static constexpr bool legacy_ok = lds_plane <= 160 * 1024 && lds_col <= 160 * 1024;
+    using PG = P2Geom<N1, N2>;
+    using CG = C2Geom<N0, TC>;
+    static constexpr size_t lds2_plane = (size_t)(PG::LDSN + N1 + N2) * sizeof(cpx<T>);
+    static constexpr size_t lds2_col = (size_t)(CG::LDSN + N0) * sizeof(cpx<T>) + (size_t)N0 * sizeof(T);
+    static constexpr bool two_ok = PG::ok && CG::ok && lds2_plane <= 160 * 1024 && lds2_col <= 160 * 1024;
+    static_assert(legacy_ok || two_ok, "no kernel fits in LDS for this grid");
     static_assert(fft_len_supported(N0) && fft_len_supported(N1) && fft_len_supported(N2), "radix");
 
-    static hipError_t setup()
+    static hipError_t setup_legacy()
     {
-        hipError_t e;
-        e = hipFuncSetAttribute((const void*)k_plane_end<T, N1, N2, NTP>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_plane);
-        if (e != hipSuccess) return e;
-        e = hipFuncSetAttribute((const void*)k_plane_mid<T, N1, N2, NTP>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_plane);
-        if (e != hipSuccess) return e;
-        return hipFuncSetAttribute((const void*)k_column<T, N0, N1, N2, TC, NTC>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_col);
+        if constexpr (legacy_ok) {
+            hipError_t e;
+            e = hipFuncSetAttribute((const void*)k_plane_end<T, N1, N2, NTP>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_plane);
+            if (e != hipSuccess) return e;
+            e = hipFuncSetAttribute((const void*)k_plane_mid<T, N1, N2, NTP>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_plane);
+            if (e != hipSuccess) return e;
+            return hipFuncSetAttribute((const void*)k_column<T, N0, N1, N2, TC, NTC>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_col);
+        }
+        return hipErrorNotSupported;
     }
-    static void begin(const PropArgs<T>& a, hipStream_t st)
+    static void begin_legacy(const PropArgs<T>& a, hipStream_t st)
     {
-        hipLaunchKernelGGL((k_plane_end<T, N1, N2, NTP>), dim3(N0, a.nb), dim3(NTP), lds_plane, st, a, 0,
-                           (int)PL_BEGIN);
+        if constexpr (legacy_ok)
+            hipLaunchKernelGGL((k_plane_end<T, N1, N2, NTP>), dim3(N0, a.nb), dim3(NTP), lds_plane, st, a, 0,
+                               (int)PL_BEGIN);
     }
-    static void step(const PropArgs<T>& a, int s, hipStream_t st)
+    static void step_legacy(const PropArgs<T>& a, int s, hipStream_t st)
     {
-        hipLaunchKernelGGL((k_column<T, N0, N1, N2, TC, NTC>), dim3(ntile, 2 * a.nb), dim3(NTC), lds_col, st, a,
-                           a.nb, a.nb);
-        hipLaunchKernelGGL((k_plane_mid<T, N1, N2, NTP>), dim3(N0, a.nb), dim3(NTP), lds_plane, st, a, s);
-        hipLaunchKernelGGL((k_column<T, N0, N1, N2, TC, NTC>), dim3(ntile, a.nb), dim3(NTC), lds_col, st, a,
-                           a.nb, 0);
-        hipLaunchKernelGGL((k_plane_end<T, N1, N2, NTP>), dim3(N0, a.nb), dim3(NTP), lds_plane, st, a, s,
-                           (int)(s == a.N - 1 ? PL_LAST : PL_STEP));
+        if constexpr (legacy_ok) {
+            hipLaunchKernelGGL((k_column<T, N0, N1, N2, TC, NTC>), dim3(ntile, 2 * a.nb), dim3(NTC), lds_col, st,
+                               a, a.nb, a.nb);
+            hipLaunchKernelGGL((k_plane_mid<T, N1, N2, NTP>), dim3(N0, a.nb), dim3(NTP), lds_plane, st, a, s);
+            hipLaunchKernelGGL((k_column<T, N0, N1, N2, TC, NTC>), dim3(ntile, a.nb), dim3(NTC), lds_col, st, a,
+                               a.nb, 0);
+            hipLaunchKernelGGL((k_plane_end<T, N1, N2, NTP>), dim3(N0, a.nb), dim3(NTP), lds_plane, st, a, s,
+                               (int)(s == a.N - 1 ? PL_LAST : PL_STEP));
+        }
     }
-    static KernelSet<T> make() { return KernelSet<T>{N0, N1, N2, &setup, &begin, &step}; }
+    static hipError_t setup_two()
+    {
+        if constexpr (two_ok) {
+            hipError_t e;
+            e = hipFuncSetAttribute((const void*)k_plane_end2<T, N1, N2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds2_plane);
+            if (e != hipSuccess) return e;
+            e = hipFuncSetAttribute((const void*)k_plane_mid2<T, N1, N2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds2_plane);
+            if (e != hipSuccess) return e;
+            return hipFuncSetAttribute((const void*)k_column2<T, N0, N1, N2, TC>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2_col);
+        }
+        return hipErrorNotSupported;
+    }
+    static void begin_two(const PropArgs<T>& a, hipStream_t st)
+    {
+        if constexpr (two_ok)
+            hipLaunchKernelGGL((k_plane_end2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, 0,
+                               (int)PL_BEGIN);
+    }
+    static void step_two(const PropArgs<T>& a, int s, hipStream_t st)
+    {
+        if constexpr (two_ok) {
+            hipLaunchKernelGGL((k_column2<T, N0, N1, N2, TC>), dim3(ntile, 2 * a.nb), dim3(CG::NT), lds2_col, st, a,
+                               a.nb, a.nb);
+            hipLaunchKernelGGL((k_plane_mid2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, s);
+            hipLaunchKernelGGL((k_column2<T, N0, N1, N2, TC>), dim3(ntile, a.nb), dim3(CG::NT), lds2_col, st, a,
+                               a.nb, 0);
+            hipLaunchKernelGGL((k_plane_end2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, s,
+                               (int)(s == a.N - 1 ? PL_LAST : PL_STEP));
+        }
+    }
+    static KernelSet<T> make()
+    {
+        KernelSet<T> k{N0, N1, N2, &setup_legacy, &begin_legacy, &step_legacy, nullptr, nullptr, nullptr, "stockham"};
+        if (two_ok) {
+            k.setup2 = &setup_two;
+            k.begin2 = &begin_two;
+            k.step2 = &step_two;
+        }
+        if (!legacy_ok) {
+            k.setup = &setup_two;
+            k.begin = &begin_two;
+            k.step = &step_two;
+        }
+        return k;
+    }
 };
 
 #define SCFT_GRID_COMMON(T)                                                                         \
@@ -92,6 +155,9 @@ PropEngine<T>::~PropEngine()
         if (p) (void)hipFree(p);
     for (hipEvent_t e : ev_)
         if (e) (void)hipEventDestroy(e);
+    if (fork_) (void)hipEventDestroy(fork_);
+    if (join_) (void)hipEventDestroy(join_);
+    if (st2_) (void)hipStreamDestroy(st2_);
 }
 
 template <typename T>
@@ -137,13 +203,21 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
     nbmax_ = nbmax;
     M_ = (long long)m[0] * m[1] * m[2];
     ks_ = find_kernels<T>(m[0], m[1], m[2]);
+    if (ks_) {
+        const char* legacy = getenv("SCFT_LEGACY_FFT");
+        const bool use_two = ks_->step2 && !(legacy && legacy[0] == '1');
+        setup_ = use_two ? ks_->setup2 : ks_->setup;
+        begin_ = use_two ? ks_->begin2 : ks_->begin;
+        step_ = use_two ? ks_->step2 : ks_->step;
+        path_ = use_two ? "two-stage" : "stockham";
+    }
     if (!ks_) {
         char b[256];
         snprintf(b, sizeof b, "no %s propagator kernels for grid %dx%dx%d (supported: %s)",
                  sizeof(T) == 8 ? "FP64" : "FP32", m[0], m[1], m[2], supported_grids(sizeof(T) == 8).c_str());
         return b;
     }
-    hipError_t e = ks_->setup();
+    hipError_t e = setup_();
     if (e != hipSuccess) return std::string("hipFuncSetAttribute: ") + hipGetErrorString(e);
     nblk_ = (int)std::min<long long>(1024, (M_ + RED_NT - 1) / RED_NT);
     auto alloc = [&](void** p, size_t bytes) -> bool {
@@ -166,6 +240,15 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
     }
     e = set_box(D);
     if (e != hipSuccess) return std::string("kinetic tables: ") + hipGetErrorString(e);
+    const char* ns = getenv("SCFT_STREAMS");
+    nstreams_ = ns ? atoi(ns) : 2;
+    if (nstreams_ > 1) {
+        // second stream at the default priority, plus fork/join events (no timing)
+        if (hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&fork_, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&join_, hipEventDisableTiming) != hipSuccess)
+            return "stream / event creation failed";
+    }
     return "";
 }
 
@@ -221,8 +304,36 @@ hipError_t PropEngine<T>::run(const double* dW, long long wstride, double* dphi,
     hipLaunchKernelGGL((k_factors<T>), rg, dim3(RED_NT), 0, st, dW, wstride, d_slot2rep, M_, cc.ds, cAB_, e4A_,
                        e4B_);
     if (timing_) (void)hipEventRecord(ev_[0], st);
-    ks_->begin(a, st);
-    for (int s = 0; s < cc.N; ++s) ks_->step(a, s, st);
+    if (nstreams_ > 1 && nb >= 2) {
+        // Two independent replica groups on two streams: one group's LDS/latency-bound plane
+        // kernels co-run with the other group's bandwidth-bound column kernels.
+        PropArgs<T> b = a;
+        const int na = (nb + 1) / 2;
+        a.nb = na;
+        b.nb = nb - na;
+        const size_t off = (size_t)na * (size_t)M_;
+        b.Ch = Ch_ + off;
+        b.Cf = Cf_ + off;
+        b.e4A = e4A_ + off;
+        b.e4B = e4B_ + off;
+        b.qs = qs_ + off * (size_t)(cc.N + 1);
+        hipError_t e = hipEventRecord(fork_, st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st2_, fork_, 0);
+        if (e != hipSuccess) return e;
+        begin_(a, st);
+        begin_(b, st2_);
+        for (int s = 0; s < cc.N; ++s) {
+            step_(a, s, st);
+            step_(b, s, st2_);
+        }
+        e = hipEventRecord(join_, st2_);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, join_, 0);
+        if (e != hipSuccess) return e;
+        a.nb = nb;
+    } else {
+        begin_(a, st);
+        for (int s = 0; s < cc.N; ++s) step_(a, s, st);
+    }
     if (timing_) (void)hipEventRecord(ev_[1], st);
     hipLaunchKernelGGL((k_qsum<T>), rg, dim3(RED_NT), 0, st, qs_, M_, cc.N, cc.Nh, part_);
     hipLaunchKernelGGL(k_scalars, dim3(nb), dim3(RED_NT), 0, st, part_, nblk_, M_, cc, cAB_, scal_);

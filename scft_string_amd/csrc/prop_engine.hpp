@@ -15,6 +15,11 @@ struct KernelSet {
     hipError_t (*setup)();
     void (*begin)(const PropArgs<T>&, hipStream_t);
     void (*step)(const PropArgs<T>&, int s, hipStream_t);
+    // two-stage register-FFT variant (nullptr when the grid has none)
+    hipError_t (*setup2)();
+    void (*begin2)(const PropArgs<T>&, hipStream_t);
+    void (*step2)(const PropArgs<T>&, int s, hipStream_t);
+    const char* name;
 };
 
 // kernel set for an (m0, m1, m2) grid, or nullptr when no instantiation exists
@@ -52,6 +57,7 @@ public:
     int N() const { return N_; }
     int nbmax() const { return nbmax_; }
     size_t bytes_allocated() const { return bytes_; }
+    const char* fft_path() const { return path_; }
     // HIP events around the contour loop of the next run() calls (diagnostics)
     hipError_t enable_timing(bool on);
     // elapsed ms of the contour loop of the last run() (after the stream is synchronised)
@@ -63,6 +69,10 @@ private:
     long long M_ = 0;
     int N_ = 0, nbmax_ = 0, nblk_ = 0;
     const KernelSet<T>* ks_ = nullptr;
+    hipError_t (*setup_)() = nullptr;
+    void (*begin_)(const PropArgs<T>&, hipStream_t) = nullptr;
+    void (*step_)(const PropArgs<T>&, int s, hipStream_t) = nullptr;
+    const char* path_ = "";
     cpx<T>* Ch_ = nullptr;
     cpx<T>* Cf_ = nullptr;
     T* e4A_ = nullptr;
@@ -76,6 +86,9 @@ private:
     size_t bytes_ = 0;
     bool timing_ = false;
     hipEvent_t ev_[2] = {nullptr, nullptr};
+    int nstreams_ = 1;
+    hipStream_t st2_ = nullptr;
+    hipEvent_t fork_ = nullptr, join_ = nullptr;
 };
 
 }  // namespace scft

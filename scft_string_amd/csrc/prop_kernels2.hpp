@@ -1,0 +1,525 @@
+// prop_kernels2.hpp -- propagator kernels built on the two-stage line FFT (fft2.hpp).
+//
+// Same contour-step schedule and data layout as prop_kernels.hpp (col(Ch,Cf) -> plane_mid ->
+// col(Ch) -> plane_end, one complex field q + i q-dagger per slot), but every 1-D transform
+// is two register DFTs around a single LDS transpose. The first and last stages read and
+// write global memory directly, and inverse -> pointwise -> forward never leaves
+// registers. Used for grids whose three axes all split as N1 x N2 with 1 < N1 <= N2 <= 16
+// (24 ... 128); smaller axes use the LDS Stockham kernels.
+#pragma once
+#include "fft2.hpp"
+#include "prop_types.hpp"
+
+namespace scft {
+
+constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+// Plane geometry: rows y (NY), columns z (NZ). Work items per stage:
+//   Z1 (y, a1)  holds x[y][a1 + A1 a2], a2 < A2      (real-space ownership)
+//   Z2 (y, k2)  holds Z[y][k2 + A2 k1], k1 < A1      (after the z transform)
+//   Y1 (kz, b1) holds P[b1 + B1 b2][kz], b2 < B2
+//   Y2 (kz, l2) holds Y[l2 + B2 l1][kz], l1 < B1     (k-space ownership, global layout)
+template <int NY, int NZ>
+struct P2Geom {
+    static constexpr int A1 = split_n1(NZ), A2 = NZ / A1;
+    static constexpr int B1 = split_n1(NY), B2 = NY / B1;
+    static constexpr int NT = cmax(64, cdiv(NY * A1, 64) * 64) > 1024 ? 1024 : cmax(64, cdiv(NY * A1, 64) * 64);
+    static constexpr int IZ1 = cdiv(NY * A1, NT), IZ2 = cdiv(NY * A2, NT);
+    static constexpr int IY1 = cdiv(NZ * B1, NT), IY2 = cdiv(NZ * B2, NT);
+    // LDS layouts in complex elements
+    static constexpr int PZ = A2 + 1, RZ = A1 * PZ + 4;   // TZ: y*RZ + a1*PZ + k2
+    static constexpr int PP = NZ + 8;                     // PL: y*PP + kz
+    static constexpr int PY = B2, CY = B1 * PY + 1;       // TY: kz*CY + b1*PY + l2
+    static constexpr int LDSN = cmax(cmax(NY * RZ, NY * PP), NZ * CY);
+    static constexpr bool ok = A1 > 1 && B1 > 1;
+};
+
+template <typename T, int NY, int NZ>
+struct P2 : P2Geom<NY, NZ> {
+    using G = P2Geom<NY, NZ>;
+    using G::A1; using G::A2; using G::B1; using G::B2; using G::NT;
+    using G::IZ1; using G::IZ2; using G::IY1; using G::IY2;
+    using G::PZ; using G::RZ; using G::PP; using G::PY; using G::CY;
+
+    // forward 2-D transform of the Z1-layout registers zr; result to global `out` (row-major
+    // [y][z] of the plane). buf must be free on entry; it is in use on return.
+    static __device__ __forceinline__ void fwd(cpx<T> (&zr)[IZ1][A2], cpx<T>* buf, const cpx<T>* twz,
+                                               const cpx<T>* twy, cpx<T>* __restrict__ out, int tid)
+    {
+        // z stage 1
+#pragma unroll
+        for (int it = 0; it < IZ1; ++it) {
+            const int w = tid + it * NT;
+            if ((NY * A1) % NT == 0 || w < NY * A1) {
+                const int y = w / A1, a1 = w % A1;
+                rdft<T, A2, false>(zr[it]);
+                sfor<A2>([&](auto kc) {
+                    constexpr int k2 = decltype(kc)::value;
+                    const cpx<T> v = k2 == 0 ? zr[it][0] : cmul(zr[it][k2], twz[(a1 * k2) % NZ]);
+                    buf[y * RZ + a1 * PZ + k2] = v;
+                });
+            }
+        }
+        __syncthreads();
+        // z stage 2
+        cpx<T> zz[IZ2][A1];
+#pragma unroll
+        for (int it = 0; it < IZ2; ++it) {
+            const int w = tid + it * NT;
+            if ((NY * A2) % NT == 0 || w < NY * A2) {
+                const int y = w / A2, k2 = w % A2;
+                sfor<A1>([&](auto ac) {
+                    constexpr int a1 = decltype(ac)::value;
+                    zz[it][a1] = buf[y * RZ + a1 * PZ + k2];
+                });
+                rdft<T, A1, false>(zz[it]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IZ2; ++it) {
+            const int w = tid + it * NT;
+            if ((NY * A2) % NT == 0 || w < NY * A2) {
+                const int y = w / A2, k2 = w % A2;
+                sfor<A1>([&](auto kc) {
+                    constexpr int k1 = decltype(kc)::value;
+                    buf[y * PP + k2 + A2 * k1] = zz[it][k1];
+                });
+            }
+        }
+        __syncthreads();
+        // y stage 1
+        cpx<T> yy[IY1][B2];
+#pragma unroll
+        for (int it = 0; it < IY1; ++it) {
+            const int w = tid + it * NT;
+            if ((NZ * B1) % NT == 0 || w < NZ * B1) {
+                const int kz = w % NZ, b1 = w / NZ;
+                sfor<B2>([&](auto bc) {
+                    constexpr int b2 = decltype(bc)::value;
+                    yy[it][b2] = buf[(b1 + B1 * b2) * PP + kz];
+                });
+                rdft<T, B2, false>(yy[it]);
+                sfor<B2>([&](auto lc) {
+                    constexpr int l2 = decltype(lc)::value;
+                    if (l2 != 0) yy[it][l2] = cmul(yy[it][l2], twy[(b1 * l2) % NY]);
+                });
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IY1; ++it) {
+            const int w = tid + it * NT;
+            if ((NZ * B1) % NT == 0 || w < NZ * B1) {
+                const int kz = w % NZ, b1 = w / NZ;
+                sfor<B2>([&](auto lc) {
+                    constexpr int l2 = decltype(lc)::value;
+                    buf[kz * CY + b1 * PY + l2] = yy[it][l2];
+                });
+            }
+        }
+        __syncthreads();
+        // y stage 2 -> global
+#pragma unroll
+        for (int it = 0; it < IY2; ++it) {
+            const int w = tid + it * NT;
+            if ((NZ * B2) % NT == 0 || w < NZ * B2) {
+                const int kz = w % NZ, l2 = w / NZ;
+                cpx<T> u[B1];
+                sfor<B1>([&](auto bc) {
+                    constexpr int b1 = decltype(bc)::value;
+                    u[b1] = buf[kz * CY + b1 * PY + l2];
+                });
+                rdft<T, B1, false>(u);
+                sfor<B1>([&](auto lc) {
+                    constexpr int l1 = decltype(lc)::value;
+                    out[(l2 + B2 * l1) * NZ + kz] = u[l1];
+                });
+            }
+        }
+    }
+
+    // k-space input of the inverse, in Y2 ownership, read straight from global memory
+    static __device__ __forceinline__ void inv_load(const cpx<T>* __restrict__ in, cpx<T> (&u)[IY2][B1], int tid)
+    {
+#pragma unroll
+        for (int it = 0; it < IY2; ++it) {
+            const int w = tid + it * NT;
+            if ((NZ * B2) % NT == 0 || w < NZ * B2) {
+                const int kz = w % NZ, l2 = w / NZ;
+                sfor<B1>([&](auto lc) {
+                    constexpr int l1 = decltype(lc)::value;
+                    u[it][l1] = in[(l2 + B2 * l1) * NZ + kz];
+                });
+            }
+        }
+    }
+
+    // inverse 2-D transform from global `in` to Z1-layout registers zr.
+    // buf must be free on entry; it is in use on return.
+    static __device__ __forceinline__ void inv(const cpx<T>* __restrict__ in, cpx<T>* buf, const cpx<T>* twz,
+                                               const cpx<T>* twy, cpx<T> (&zr)[IZ1][A2], int tid)
+    {
+        cpx<T> u[IY2][B1];
+        inv_load(in, u, tid);
+        inv_regs(u, buf, twz, twy, zr, tid);
+    }
+
+    // inverse 2-D transform of Y2-ownership registers u (from inv_load) to Z1-layout zr
+    static __device__ __forceinline__ void inv_regs(cpx<T> (&uin)[IY2][B1], cpx<T>* buf, const cpx<T>* twz,
+                                                    const cpx<T>* twy, cpx<T> (&zr)[IZ1][A2], int tid)
+    {
+        // y stage 2'
+#pragma unroll
+        for (int it = 0; it < IY2; ++it) {
+            const int w = tid + it * NT;
+            if ((NZ * B2) % NT == 0 || w < NZ * B2) {
+                const int kz = w % NZ, l2 = w / NZ;
+                cpx<T>* u = uin[it];
+                rdft<T, B1, true>(u);
+                sfor<B1>([&](auto bc) {
+                    constexpr int b1 = decltype(bc)::value;
+                    const cpx<T> v = (b1 == 0 || l2 == 0) ? u[b1] : cmulc(u[b1], twy[(b1 * l2) % NY]);
+                    buf[kz * CY + b1 * PY + l2] = v;
+                });
+            }
+        }
+        __syncthreads();
+        // y stage 1'
+        cpx<T> yy[IY1][B2];
+#pragma unroll
+        for (int it = 0; it < IY1; ++it) {
+            const int w = tid + it * NT;
+            if ((NZ * B1) % NT == 0 || w < NZ * B1) {
+                const int kz = w % NZ, b1 = w / NZ;
+                sfor<B2>([&](auto lc) {
+                    constexpr int l2 = decltype(lc)::value;
+                    yy[it][l2] = buf[kz * CY + b1 * PY + l2];
+                });
+                rdft<T, B2, true>(yy[it]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IY1; ++it) {
+            const int w = tid + it * NT;
+            if ((NZ * B1) % NT == 0 || w < NZ * B1) {
+                const int kz = w % NZ, b1 = w / NZ;
+                sfor<B2>([&](auto bc) {
+                    constexpr int b2 = decltype(bc)::value;
+                    buf[(b1 + B1 * b2) * PP + kz] = yy[it][b2];
+                });
+            }
+        }
+        __syncthreads();
+        // z stage 2'
+        cpx<T> zz[IZ2][A1];
+#pragma unroll
+        for (int it = 0; it < IZ2; ++it) {
+            const int w = tid + it * NT;
+            if ((NY * A2) % NT == 0 || w < NY * A2) {
+                const int y = w / A2, k2 = w % A2;
+                sfor<A1>([&](auto kc) {
+                    constexpr int k1 = decltype(kc)::value;
+                    zz[it][k1] = buf[y * PP + k2 + A2 * k1];
+                });
+                rdft<T, A1, true>(zz[it]);
+                sfor<A1>([&](auto ac) {
+                    constexpr int a1 = decltype(ac)::value;
+                    if (a1 != 0) zz[it][a1] = cmulc(zz[it][a1], twz[(a1 * k2) % NZ]);
+                });
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IZ2; ++it) {
+            const int w = tid + it * NT;
+            if ((NY * A2) % NT == 0 || w < NY * A2) {
+                const int y = w / A2, k2 = w % A2;
+                sfor<A1>([&](auto ac) {
+                    constexpr int a1 = decltype(ac)::value;
+                    buf[y * RZ + a1 * PZ + k2] = zz[it][a1];
+                });
+            }
+        }
+        __syncthreads();
+        // z stage 1'
+#pragma unroll
+        for (int it = 0; it < IZ1; ++it) {
+            const int w = tid + it * NT;
+            if ((NY * A1) % NT == 0 || w < NY * A1) {
+                const int y = w / A1, a1 = w % A1;
+                sfor<A2>([&](auto kc) {
+                    constexpr int k2 = decltype(kc)::value;
+                    zr[it][k2] = buf[y * RZ + a1 * PZ + k2];
+                });
+                rdft<T, A2, true>(zr[it]);
+            }
+        }
+    }
+
+    // plane element index of Z1 item w, value a2
+    static __device__ __forceinline__ int z1_elem(int w, int a2) { return (w / A1) * NZ + (w % A1) + A1 * a2; }
+    static __device__ __forceinline__ bool z1_valid(int w) { return (NY * A1) % NT == 0 || w < NY * A1; }
+};
+
+template <typename T, int NY, int NZ>
+__device__ __forceinline__ void p2_load_tw(cpx<T>* twz, cpx<T>* twy, const PropArgs<T>& a, int tid, int nt)
+{
+    for (int i = tid; i < NZ; i += nt) twz[i] = a.tw2[i];
+    for (int i = tid; i < NY; i += nt) twy[i] = a.tw1[i];
+}
+
+// plane_end on the two-stage FFT (see k_plane_end in prop_kernels.hpp for the contract)
+template <typename T, int NY, int NZ>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT))) k_plane_end2(PropArgs<T> a, int s, int mode)
+{
+    using P = P2<T, NY, NZ>;
+    constexpr int NT = P::NT, IZ1 = P::IZ1, A2 = P::A2, NE = NY * NZ;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* twz = buf + P::LDSN;
+    cpx<T>* twy = twz + NZ;
+    const int tid = threadIdx.x;
+    const int x = blockIdx.x, slot = blockIdx.y;
+    const size_t fo = (size_t)slot * (size_t)a.M + (size_t)x * NE;
+    p2_load_tw<T, NY, NZ>(twz, twy, a, tid, NT);
+
+    // issue every global load of this plane up front: both k-space inputs and the factors
+    cpx<T> uh[P::IY2][P::B1], uf[P::IY2][P::B1];
+    if (mode != PL_BEGIN) {
+        P::inv_load(a.Ch + fo, uh, tid);
+        P::inv_load(a.Cf + fo, uf, tid);
+    }
+    T ea[IZ1][A2], eb[IZ1][A2];
+#pragma unroll
+    for (int it = 0; it < IZ1; ++it) {
+        const int w = tid + it * NT;
+        if (P::z1_valid(w)) {
+#pragma unroll
+            for (int a2 = 0; a2 < A2; ++a2) {
+                const int e = P::z1_elem(w, a2);
+                ea[it][a2] = a.e4A[fo + e];
+                eb[it][a2] = a.e4B[fo + e];
+            }
+        }
+    }
+    cpx<T> q[IZ1][A2];
+    int sout;
+    __syncthreads();  // twiddles
+    if (mode == PL_BEGIN) {
+#pragma unroll
+        for (int it = 0; it < IZ1; ++it)
+#pragma unroll
+            for (int a2 = 0; a2 < A2; ++a2) q[it][a2] = {T(1), T(1)};
+        sout = 0;
+    } else {
+        cpx<T> h1[IZ1][A2];
+        P::inv_regs(uh, buf, twz, twy, q, tid);  // q holds qs0 for now
+        __syncthreads();
+        P::inv_regs(uf, buf, twz, twy, h1, tid);
+        const bool qa = s < a.n1, pb = s < a.n2;  // species of this step (density.h:136-146)
+#pragma unroll
+        for (int it = 0; it < IZ1; ++it)
+#pragma unroll
+            for (int a2 = 0; a2 < A2; ++a2) {
+                const T ex = qa ? ea[it][a2] : eb[it][a2];
+                const T ey = pb ? eb[it][a2] : ea[it][a2];
+                q[it][a2].x = (q[it][a2].x * ex * T(4) - h1[it][a2].x * (ex * ex)) / T(3);
+                q[it][a2].y = (q[it][a2].y * ey * T(4) - h1[it][a2].y * (ey * ey)) / T(3);
+            }
+        sout = s + 1;
+    }
+    cpx<T>* qo = a.qs + ((size_t)slot * (size_t)(a.N + 1) + (size_t)sout) * (size_t)a.M + (size_t)x * NE;
+#pragma unroll
+    for (int it = 0; it < IZ1; ++it) {
+        const int w = tid + it * NT;
+        if (P::z1_valid(w))
+#pragma unroll
+            for (int a2 = 0; a2 < A2; ++a2) qo[P::z1_elem(w, a2)] = q[it][a2];
+    }
+    if (mode == PL_LAST) return;
+
+    const bool qa = sout < a.n1, pb = sout < a.n2;
+    cpx<T> v[IZ1][A2];
+#pragma unroll
+    for (int it = 0; it < IZ1; ++it)
+#pragma unroll
+        for (int a2 = 0; a2 < A2; ++a2) {
+            const T ex = qa ? ea[it][a2] : eb[it][a2];
+            const T ey = pb ? eb[it][a2] : ea[it][a2];
+            v[it][a2] = {q[it][a2].x * ex, q[it][a2].y * ey};
+            q[it][a2] = {q[it][a2].x * (ex * ex), q[it][a2].y * (ey * ey)};
+        }
+    __syncthreads();
+    P::fwd(v, buf, twz, twy, a.Ch + fo, tid);
+    __syncthreads();
+    P::fwd(q, buf, twz, twy, a.Cf + fo, tid);
+}
+
+// plane_mid on the two-stage FFT: Ch <- F2( E1/2 * F2^-1(Ch) )
+template <typename T, int NY, int NZ>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT))) k_plane_mid2(PropArgs<T> a, int s)
+{
+    using P = P2<T, NY, NZ>;
+    constexpr int NT = P::NT, IZ1 = P::IZ1, A2 = P::A2, NE = NY * NZ;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* twz = buf + P::LDSN;
+    cpx<T>* twy = twz + NZ;
+    const int tid = threadIdx.x;
+    const int x = blockIdx.x, slot = blockIdx.y;
+    const size_t fo = (size_t)slot * (size_t)a.M + (size_t)x * NE;
+    p2_load_tw<T, NY, NZ>(twz, twy, a, tid, NT);
+    const bool qa = s < a.n1, pb = s < a.n2;
+    T ex[IZ1][A2], ey[IZ1][A2];
+#pragma unroll
+    for (int it = 0; it < IZ1; ++it) {
+        const int w = tid + it * NT;
+        if (P::z1_valid(w)) {
+#pragma unroll
+            for (int a2 = 0; a2 < A2; ++a2) {
+                const int e = P::z1_elem(w, a2);
+                const T va = a.e4A[fo + e], vb = a.e4B[fo + e];
+                const T x4 = qa ? va : vb, y4 = pb ? vb : va;
+                ex[it][a2] = x4 * x4;
+                ey[it][a2] = y4 * y4;
+            }
+        }
+    }
+    __syncthreads();  // twiddles
+    cpx<T> z[IZ1][A2];
+    P::inv(a.Ch + fo, buf, twz, twy, z, tid);
+#pragma unroll
+    for (int it = 0; it < IZ1; ++it)
+#pragma unroll
+        for (int a2 = 0; a2 < A2; ++a2) {
+            z[it][a2].x *= ex[it][a2];
+            z[it][a2].y *= ey[it][a2];
+        }
+    __syncthreads();
+    P::fwd(z, buf, twz, twy, a.Ch + fo, tid);
+}
+
+// Column pass on the two-stage FFT: for TC consecutive (y,z) columns of one field, forward
+// FFT along x, multiply by the kinetic factor, inverse FFT along x (see k_column).
+template <int N0, int TC>
+struct C2Geom {
+    static constexpr int X1 = split_n1(N0), X2 = N0 / X1;
+    static constexpr int NT = cdiv(TC * X1, 64) * 64 > 1024 ? 1024 : cdiv(TC * X1, 64) * 64;
+    static constexpr int I1 = cdiv(TC * X1, NT), I2 = cdiv(TC * X2, NT);
+    static constexpr int PX = X2, CX = X1 * PX + 1;  // TX: c*CX + n1*PX + k2
+    static constexpr int LDSN = TC * CX;
+    static constexpr bool ok = X1 > 1;
+};
+
+template <typename T, int N0, int NY, int NZ, int TC>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<N0, TC>::NT))) k_column2(PropArgs<T> a, int nh, int nf)
+{
+    using G = C2Geom<N0, TC>;
+    constexpr int NT = G::NT, X1 = G::X1, X2 = G::X2, I1 = G::I1, I2 = G::I2, CX = G::CX, PX = G::PX;
+    constexpr int NC = NY * NZ;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* tw0 = buf + G::LDSN;
+    T* gx = reinterpret_cast<T*>(tw0 + N0);
+    const int tid = threadIdx.x;
+    const int tile = blockIdx.x, f = blockIdx.y;
+    const bool full = f >= nh;
+    cpx<T>* fld = full ? a.Cf + (size_t)(f - nh) * (size_t)a.M : a.Ch + (size_t)f * (size_t)a.M;
+    const int c0 = tile * TC;
+    for (int i = tid; i < N0; i += NT) {
+        tw0[i] = a.tw0[i];
+        gx[i] = a.gx[i];
+    }
+    cpx<T> v[I1][X2];
+    // stage 1: load rows n1 + X1 n2 straight from global
+#pragma unroll
+    for (int it = 0; it < I1; ++it) {
+        const int w = tid + it * NT;
+        if ((TC * X1) % NT == 0 || w < TC * X1) {
+            const int c = w % TC, n1 = w / TC, col = c0 + c;
+            const bool in = NC % TC == 0 || col < NC;
+            sfor<X2>([&](auto nc) {
+                constexpr int n2 = decltype(nc)::value;
+                v[it][n2] = in ? fld[(size_t)(n1 + X1 * n2) * NC + col] : cpx<T>{T(0), T(0)};
+            });
+        }
+    }
+    __syncthreads();  // twiddle / gx tables
+#pragma unroll
+    for (int it = 0; it < I1; ++it) {
+        const int w = tid + it * NT;
+        if ((TC * X1) % NT == 0 || w < TC * X1) {
+            const int c = w % TC, n1 = w / TC;
+            rdft<T, X2, false>(v[it]);
+            sfor<X2>([&](auto kc) {
+                constexpr int k2 = decltype(kc)::value;
+                buf[c * CX + n1 * PX + k2] = k2 == 0 ? v[it][0] : cmul(v[it][k2], tw0[(n1 * k2) % N0]);
+            });
+        }
+    }
+    __syncthreads();
+    cpx<T> u[I2][X1];
+#pragma unroll
+    for (int it = 0; it < I2; ++it) {
+        const int w = tid + it * NT;
+        if ((TC * X2) % NT == 0 || w < TC * X2) {
+            const int c = w % TC, k2 = w / TC, col = c0 + c;
+            sfor<X1>([&](auto nc) {
+                constexpr int n1 = decltype(nc)::value;
+                u[it][n1] = buf[c * CX + n1 * PX + k2];
+            });
+            rdft<T, X1, false>(u[it]);
+            // kinetic factor at kx = k2 + X2 k1 (exp(-k^2 ds/12)/M, or its square * M)
+            const int y = (col < NC ? col : 0) / NZ, z = (col < NC ? col : 0) % NZ;
+            const T gyz = a.gy[y] * a.gz[z];
+            sfor<X1>([&](auto kc) {
+                constexpr int k1 = decltype(kc)::value;
+                T g = gx[k2 + X2 * k1] * gyz;
+                if (full) g = g * g;
+                g = g * a.invM;
+                u[it][k1].x *= g;
+                u[it][k1].y *= g;
+            });
+            rdft<T, X1, true>(u[it]);
+            sfor<X1>([&](auto nc) {
+                constexpr int n1 = decltype(nc)::value;
+                if (n1 != 0) u[it][n1] = cmulc(u[it][n1], tw0[(n1 * k2) % N0]);
+            });
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < I2; ++it) {
+        const int w = tid + it * NT;
+        if ((TC * X2) % NT == 0 || w < TC * X2) {
+            const int c = w % TC, k2 = w / TC;
+            sfor<X1>([&](auto nc) {
+                constexpr int n1 = decltype(nc)::value;
+                buf[c * CX + n1 * PX + k2] = u[it][n1];
+            });
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < I1; ++it) {
+        const int w = tid + it * NT;
+        if ((TC * X1) % NT == 0 || w < TC * X1) {
+            const int c = w % TC, n1 = w / TC, col = c0 + c;
+            sfor<X2>([&](auto kc) {
+                constexpr int k2 = decltype(kc)::value;
+                v[it][k2] = buf[c * CX + n1 * PX + k2];
+            });
+            rdft<T, X2, true>(v[it]);
+            if (NC % TC == 0 || col < NC)
+                sfor<X2>([&](auto nc) {
+                    constexpr int n2 = decltype(nc)::value;
+                    fld[(size_t)(n1 + X1 * n2) * NC + col] = v[it][n2];
+                });
+        }
+    }
+}
+
+}  // namespace scft

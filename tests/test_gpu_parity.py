@@ -167,3 +167,42 @@ def test_free_energy_matches_oracle():
     for key in ("FEs", "FEs0", "FEs1", "phctot", "alf"):
         np.testing.assert_allclose(fe_g[key], fe_o[key], rtol=TOL64, atol=1e-12, err_msg=key)
     assert abs(fe_g["FE"] - fe_o["FE"]) <= TOL64 * abs(fe_o["FE"])
+
+
+LARGE = [
+    # m, D, N, f, alpha, ens, phidb, precision, tol (relative, vs the FP64 oracle)
+    ((48, 48, 48), (5.0, 5.0, 5.0), 12, 0.5, 0.1, 1, 0.8, "fp64", TOL64),
+    ((64, 64, 64), (6.0, 6.0, 6.0), 16, 0.45, 0.125, 1, 0.9, "fp64", TOL64),
+    ((96, 96, 96), (8.0, 8.0, 8.0), 8, 0.5, 0.125, 2, 0.7, "fp64", TOL64),
+    ((128, 128, 128), (8.0, 8.0, 8.0), 8, 0.5, 0.125, 1, 0.8, "fp32", 2e-5),
+]
+
+
+@pytest.mark.parametrize("case", LARGE, ids=lambda c: f"{c[0][0]}cube_{c[7]}")
+def test_props_large_grids_vs_oracle(case):
+    m, D, N, f, alpha, ens, phidb, prec, tol = case
+    _say("large props", m, prec)
+    W = _fields(m, D, seed=17, amp=1.5)[None]
+    ref = orc.Grid(m, D).props(W[0], N=N, alpha=alpha, phic=phidb, f=f, ens=ens)
+    with Context(m, D, N, f, alpha, ens, phidb, chi=15.0, nrep=1, precision=prec) as ctx:
+        phi, lnQ, phis = ctx.props_batch(W)
+    dl = abs(lnQ[0] - ref["lnQ"]) / max(1.0, abs(ref["lnQ"]))
+    dphi = max(_rel(phi[0, c], ref["phi"][c]) for c in range(4))
+    _say(f"  {prec} {m}: rel dlnQ={dl:.2e} rel dphi={dphi:.2e}")
+    assert dl <= tol and dphi <= tol
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_two_stage_and_stockham_paths_agree(prec, monkeypatch):
+    """The two-stage register FFT and the LDS Stockham kernels compute the same props."""
+    m, D = (32, 32, 32), (4.0, 4.0, 4.0)
+    N, f, alpha, ens, phidb = 24, 0.5, 0.125, 1, 0.8
+    W = _fields(m, D, seed=8)[None]
+    out = {}
+    for legacy in ("0", "1"):
+        monkeypatch.setenv("SCFT_LEGACY_FFT", legacy)
+        with Context(m, D, N, f, alpha, ens, phidb, chi=15.0, nrep=1, precision=prec) as ctx:
+            out[legacy] = ctx.props_batch(W)
+    tol = 1e-12 if prec == "fp64" else 1e-5
+    assert _rel(out["0"][0], out["1"][0]) <= tol
+    assert abs(out["0"][1][0] - out["1"][1][0]) <= tol * max(1.0, abs(out["1"][1][0]))
