@@ -69,13 +69,80 @@ def cpu_baseline(m, D, N, R_active, props_per_iter, budget_s=20.0):
     while True:
         g.props(W, N=N, alpha=CFG["alpha"], phic=CFG["phidb"], f=CFG["f"], ens=CFG["ens"])
         n += 1
-        if time.perf_counter() - t0 > budget_s * 0.5 or n >= 3:
+        if time.perf_counter() - t0 > budget_s * 0.5 or n >= 40:
             break
     dt = (time.perf_counter() - t0) / n
     return {"value": 1.0 / (dt * R_active * props_per_iter), "unit": "string iters/s", "cores": threads,
             "kind": "port",
             "sample": f"{n} FP64 props of one replica at {m[0]}^3, N={N} ({dt:.2f} s each), scaled to "
                       f"{R_active} replicas x {props_per_iter} props per iteration"}
+
+
+def free_energies(W, phi, lnQ, chi):
+    """FEs of every row from props outputs (scft.cu:820-829), in FP64."""
+    R, M = phi.shape[0], phi.shape[-1]
+    W = W.reshape(R, 2, M)
+    wa, wb = W[:, 0] + W[:, 1], W[:, 1] - W[:, 0]
+    pa, pb = phi[:, 0], phi[:, 1]
+    return -lnQ + np.mean(chi * pa * pb - wa * pa - wb * pb, axis=1)
+
+
+def precision_report(W, Wshape_m, c, oracle_rows=(0,)):
+    """FP32 vs FP64 GPU props on the identical (evolved) string state, plus the FP64 oracle
+    on a few rows: |dF| per replica and max |d phiA|."""
+    from scft_string_amd import Context
+
+    R = W.shape[0]
+    out = {}
+    res = {}
+    for prec in ("fp32", "fp64"):
+        with Context(c["m"], c["D"], c["N"], c["f"], c["alpha"], c["ens"], c["phidb"], c["chi"], nrep=R,
+                     precision=prec, fix=c["fix"]) as ctx:
+            phi, lnQ, _ = ctx.props_batch(W)
+        res[prec] = (phi, lnQ, free_energies(W, phi, lnQ, c["chi"]))
+    dF = np.abs(res["fp32"][2] - res["fp64"][2])
+    out["fp32_vs_fp64_max_abs_dF"] = float(dF.max())
+    out["fp32_vs_fp64_max_abs_dphiA"] = float(np.max(np.abs(res["fp32"][0][:, 0] - res["fp64"][0][:, 0])))
+    out["F_fp64"] = [float(v) for v in res["fp64"][2]]
+    try:
+        from oracle import orc
+
+        g = orc.Grid(c["m"], c["D"])
+        d = []
+        for i in oracle_rows:
+            o = g.props(W[i].reshape(2, -1), N=c["N"], alpha=c["alpha"], phic=c["phidb"], f=c["f"], ens=c["ens"])
+            Fo = free_energies(W[i:i + 1], o["phi"][None], np.array([o["lnQ"]]), c["chi"])[0]
+            d.append(abs(res["fp64"][2][i] - Fo))
+            d32 = abs(res["fp32"][2][i] - Fo)
+            out.setdefault("fp32_vs_oracle_abs_dF", []).append(float(d32))
+        out["fp64_vs_oracle_abs_dF"] = [float(v) for v in d]
+        out["oracle_rows"] = list(oracle_rows)
+    except Exception as e:  # the oracle library is test infrastructure; report if absent
+        out["oracle"] = f"unavailable: {e}"
+    return out
+
+
+def pmc_probe():
+    """Workload for rocprofv3 --pmc runs: a 512 MiB device copy of known size (the scale for
+    FETCH_SIZE / WRITE_SIZE) followed by one batched props of the benchmark string."""
+    import torch
+
+    from scft_string_amd import Context
+
+    n = 1 << 27
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    c = CFG
+    R = c["nrep"]
+    ctx = Context(c["m"], c["D"], c["N"], c["f"], c["alpha"], c["ens"], c["phidb"], c["chi"], nrep=R,
+                  precision="fp32", fix=c["fix"], dostring=1)
+    ctx.set_fields(initial_fields(c["m"], c["D"], R))
+    ctx.time_props(R - sum(c["fix"]), reps=1)
+    ctx.close()
+    print("pmc probe done", flush=True)
 
 
 def main():
@@ -85,13 +152,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--pmc-probe", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: all string replicas share it; "
                                    "multi-GPU replica sharding is a design note only (DESIGN.md)",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.pmc_probe:
+        pmc_probe()
+        return
 
     import torch
 
@@ -118,21 +190,26 @@ def main():
     print("bench: timing", a.steps, "iterations", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     props = 0
+    n_act = R - sum(c["fix"])
     for k in range(a.warmup, a.warmup + a.steps):
         rep = iteration(k)
-        props += int(np.sum(rep["steps"]))
+        props += int(np.sum(np.maximum(rep["steps"] - 1, 0)))  # steps = 1 + solve_field0's k
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     iters_per_s = a.steps / dt
 
-    # roofline of the fused contour step (4 launches per step over all active replicas)
-    n_act = R - sum(c["fix"])
+    # roofline of the contour step (all active replicas, every launch of one step)
     ms_step = ctx.time_props(n_act, reps=2)
     p = 4 if a.precision == "fp32" else 8
     bstep = algorithmic_bytes_per_step(c["m"], p) * n_act
     achieved = bstep / (ms_step * 1e-3) / 1e9
-    fe = ctx.free_energy()
+    Wend = ctx.get_fields()
     ctx.close()
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "pmc_traffic_contour_step_14x64cube_N128_fp32.json")
+    if a.precision == "fp32" and os.path.exists(tf):
+        traffic = json.load(open(tf)).get("hbm_bytes_per_contour_step")
 
     out = {
         "metric": "string iters/sec 16x64^3xN128 (batched FP32 propagator, FP64 field update)",
@@ -143,11 +220,13 @@ def main():
                                "canonical, fix=(1,1)", "replicas": R, "grid": list(c["m"]), "N": c["N"],
                    "props_per_iteration": props / a.steps, "parallelism": "1 GPU, all replicas batched"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "contour step (column, plane_mid, column, plane_end) over "
-                               f"{n_act} replicas", "ms_per_contour_step": ms_step},
-        "free_energy": {"FE": fe["FE"], "FEs": list(fe["FEs"])},
+                               f"{n_act} replicas", "ms_per_contour_step": ms_step,
+                     "algorithmic_bytes_per_contour_step": bstep},
     }
+    if not a.no_accuracy:
+        out["accuracy"] = precision_report(Wend, c["m"], c, oracle_rows=(0, R // 2))
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(c["m"], c["D"], c["N"], n_act, props / a.steps / n_act,
                                            a.cpu_budget)

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "prop_types.hpp"
 
@@ -64,6 +65,15 @@ public:
     double loop_ms() const;
 
 private:
+    hipError_t contour(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st);
+    hipError_t contour_launch(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st);
+    struct GraphEntry {
+        int nb;
+        hipStream_t st;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs_;
+    bool use_graph_ = true;
     int m_[3] = {0, 0, 0};
     double D_[3] = {0, 0, 0};
     long long M_ = 0;
@@ -86,9 +96,12 @@ private:
     size_t bytes_ = 0;
     bool timing_ = false;
     hipEvent_t ev_[2] = {nullptr, nullptr};
+    static constexpr int kMaxStreams = 4;
+    static constexpr int kDefaultStreams = 2;
     int nstreams_ = 1;
-    hipStream_t st2_ = nullptr;
-    hipEvent_t fork_ = nullptr, join_ = nullptr;
+    hipStream_t sx_[kMaxStreams - 1] = {};
+    hipEvent_t fork_ = nullptr;
+    hipEvent_t join_[kMaxStreams - 1] = {};
 };
 
 }  // namespace scft

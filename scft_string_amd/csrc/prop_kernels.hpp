@@ -135,8 +135,8 @@ __global__ void __launch_bounds__(NT) k_plane_end(PropArgs<T> a, int s, int mode
                 const cpx<T> h1 = buf[(e / N2) * P2 + (e % N2)];
                 const T ex = qa ? ea[i] : eb[i];
                 const T ey = pb ? eb[i] : ea[i];
-                q[i].x = (h0[i].x * ex * T(4) - h1.x * (ex * ex)) / T(3);
-                q[i].y = (h0[i].y * ey * T(4) - h1.y * (ey * ey)) / T(3);
+                q[i].x = Boltz<T>::rich(h0[i].x, h1.x, ex);
+                q[i].y = Boltz<T>::rich(h0[i].y, h1.y, ey);
             }
         }
         sout = s + 1;
@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(NT) k_plane_end(PropArgs<T> a, int s, int mode
         if (NE % NT == 0 || e < NE) {
             const T ex = qa ? ea[i] : eb[i];
             const T ey = pb ? eb[i] : ea[i];
-            buf[(e / N2) * P2 + (e % N2)] = {q[i].x * ex, q[i].y * ey};
+            buf[(e / N2) * P2 + (e % N2)] = {Boltz<T>::mul4(q[i].x, ex), Boltz<T>::mul4(q[i].y, ey)};
         }
     }
     __syncthreads();
@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(NT) k_plane_end(PropArgs<T> a, int s, int mode
         if (NE % NT == 0 || e < NE) {
             const T ex = qa ? ea[i] : eb[i];
             const T ey = pb ? eb[i] : ea[i];
-            buf[(e / N2) * P2 + (e % N2)] = {q[i].x * (ex * ex), q[i].y * (ey * ey)};
+            buf[(e / N2) * P2 + (e % N2)] = {Boltz<T>::mul2(q[i].x, ex), Boltz<T>::mul2(q[i].y, ey)};
         }
     }
     __syncthreads();
@@ -204,8 +204,8 @@ __global__ void __launch_bounds__(NT) k_plane_mid(PropArgs<T> a, int s)
         if (NE % NT == 0 || e < NE) {
             const T va = a.e4A[fo + e], vb = a.e4B[fo + e];
             const T x4 = qa ? va : vb, y4 = pb ? vb : va;
-            ex[i] = x4 * x4;
-            ey[i] = y4 * y4;
+            ex[i] = x4;
+            ey[i] = y4;
         }
     }
     plane_load<T, N1, N2, NT>(buf, a.Ch + fo, tid);
@@ -216,8 +216,8 @@ __global__ void __launch_bounds__(NT) k_plane_mid(PropArgs<T> a, int s)
         const int e = tid + i * NT;
         if (NE % NT == 0 || e < NE) {
             cpx<T>& v = buf[(e / N2) * P2 + (e % N2)];
-            v.x *= ex[i];
-            v.y *= ey[i];
+            v.x = Boltz<T>::mul2(v.x, ex[i]);
+            v.y = Boltz<T>::mul2(v.y, ey[i]);
         }
     }
     __syncthreads();
@@ -266,11 +266,7 @@ __global__ void __launch_bounds__(NT) k_column(PropArgs<T> a, int nh, int nf)
             const int row = e / TC, c = e % TC;
             const int col = c0 + c;
             const int y = col / N2, z = col % N2;
-            T g = gx[row] * a.gy[y < N1 ? y : 0] * a.gz[z];
-            if (full) g = g * g;
-            g = g * a.invM;
-            buf[e].x *= g;
-            buf[e].y *= g;
+            Kinetic<T>::apply(buf[e], gx[row], Kinetic<T>::col(a.gy[y < N1 ? y : 0], a.gz[z]), full, a.invM);
         }
     }
     __syncthreads();
@@ -348,7 +344,8 @@ __global__ void __launch_bounds__(RED_NT) k_wmean(const double* __restrict__ par
     }
 }
 
-// e4A = exp(-(W- + W+ - cA) ds/4), e4B = exp(-(-W- + W+ - cB) ds/4)  (GPU_kernels.h:215-225)
+// Boltzmann factors of A and B with a = (W- + W+ - cA) ds/4, (-W- + W+ - cB) ds/4
+// (GPU_kernels.h:215-225): E1/4 itself in FP64, 1 - E1/4 in FP32 (see Boltz)
 template <typename T>
 __global__ void __launch_bounds__(RED_NT) k_factors(const double* __restrict__ W, long long wstride,
                                                     const int* __restrict__ slot2rep, long long M, double ds,
@@ -360,8 +357,8 @@ __global__ void __launch_bounds__(RED_NT) k_factors(const double* __restrict__ W
     const double cA = cAB[2 * slot], cB = cAB[2 * slot + 1];
     for (long long r = (long long)blockIdx.x * RED_NT + threadIdx.x; r < M; r += (long long)gridDim.x * RED_NT) {
         const double wm = w[r], wp = w[r + M];
-        e4A[(size_t)slot * M + r] = (T)exp(-(wm + wp - cA) * ds / 4);
-        e4B[(size_t)slot * M + r] = (T)exp(-(-wm + wp - cB) * ds / 4);
+        e4A[(size_t)slot * M + r] = Boltz<T>::from_arg((wm + wp - cA) * ds / 4);
+        e4B[(size_t)slot * M + r] = Boltz<T>::from_arg((-wm + wp - cB) * ds / 4);
     }
 }
 

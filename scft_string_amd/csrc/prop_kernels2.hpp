@@ -27,10 +27,15 @@ struct P2Geom {
     static constexpr int NT = cmax(64, cdiv(NY * A1, 64) * 64) > 1024 ? 1024 : cmax(64, cdiv(NY * A1, 64) * 64);
     static constexpr int IZ1 = cdiv(NY * A1, NT), IZ2 = cdiv(NY * A2, NT);
     static constexpr int IY1 = cdiv(NZ * B1, NT), IY2 = cdiv(NZ * B2, NT);
+    // Wave-local z transposes: with A1 == A2 the lanes owning row y in the z stages are the
+    // same A1 consecutive lanes of one wave in both stages, and the TZ and PL row regions
+    // coincide, so the z-stage exchanges need no workgroup barrier.
+    static constexpr bool WLZ = A1 == A2 && 64 % A1 == 0;
     // LDS layouts in complex elements
-    static constexpr int PZ = A2 + 1, RZ = A1 * PZ + 4;   // TZ: y*RZ + a1*PZ + k2
-    static constexpr int PP = NZ + 8;                     // PL: y*PP + kz
-    static constexpr int PY = B2, CY = B1 * PY + 1;       // TY: kz*CY + b1*PY + l2
+    static constexpr int PZ = A2 + 1;                                        // TZ: y*RZ + a1*PZ + k2
+    static constexpr int RZ = WLZ ? cmax(A1 * PZ, NZ + 8) : A1 * PZ + 4;
+    static constexpr int PP = WLZ ? RZ : NZ + 8;                             // PL: y*PP + kz
+    static constexpr int PY = B2, CY = B1 * PY + 1;                          // TY: kz*CY + b1*PY + l2
     static constexpr int LDSN = cmax(cmax(NY * RZ, NY * PP), NZ * CY);
     static constexpr bool ok = A1 > 1 && B1 > 1;
 };
@@ -40,7 +45,20 @@ struct P2 : P2Geom<NY, NZ> {
     using G = P2Geom<NY, NZ>;
     using G::A1; using G::A2; using G::B1; using G::B2; using G::NT;
     using G::IZ1; using G::IZ2; using G::IY1; using G::IY2;
-    using G::PZ; using G::RZ; using G::PP; using G::PY; using G::CY;
+    using G::PZ; using G::RZ; using G::PP; using G::PY; using G::CY; using G::WLZ;
+
+    // exchange point inside the z stages: a wave-level ordering point when the exchange
+    // stays inside one wave, else a workgroup barrier
+    static __device__ __forceinline__ void zsync()
+    {
+        if constexpr (WLZ) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
+            __syncthreads();
+        }
+    }
 
     // forward 2-D transform of the Z1-layout registers zr; result to global `out` (row-major
     // [y][z] of the plane). buf must be free on entry; it is in use on return.
@@ -61,7 +79,7 @@ struct P2 : P2Geom<NY, NZ> {
                 });
             }
         }
-        __syncthreads();
+        zsync();
         // z stage 2
         cpx<T> zz[IZ2][A1];
 #pragma unroll
@@ -76,7 +94,7 @@ struct P2 : P2Geom<NY, NZ> {
                 rdft<T, A1, false>(zz[it]);
             }
         }
-        __syncthreads();
+        zsync();
 #pragma unroll
         for (int it = 0; it < IZ2; ++it) {
             const int w = tid + it * NT;
@@ -231,7 +249,7 @@ struct P2 : P2Geom<NY, NZ> {
                 });
             }
         }
-        __syncthreads();
+        zsync();
 #pragma unroll
         for (int it = 0; it < IZ2; ++it) {
             const int w = tid + it * NT;
@@ -243,7 +261,7 @@ struct P2 : P2Geom<NY, NZ> {
                 });
             }
         }
-        __syncthreads();
+        zsync();
         // z stage 1'
 #pragma unroll
         for (int it = 0; it < IZ1; ++it) {
@@ -326,8 +344,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT
             for (int a2 = 0; a2 < A2; ++a2) {
                 const T ex = qa ? ea[it][a2] : eb[it][a2];
                 const T ey = pb ? eb[it][a2] : ea[it][a2];
-                q[it][a2].x = (q[it][a2].x * ex * T(4) - h1[it][a2].x * (ex * ex)) / T(3);
-                q[it][a2].y = (q[it][a2].y * ey * T(4) - h1[it][a2].y * (ey * ey)) / T(3);
+                q[it][a2].x = Boltz<T>::rich(q[it][a2].x, h1[it][a2].x, ex);
+                q[it][a2].y = Boltz<T>::rich(q[it][a2].y, h1[it][a2].y, ey);
             }
         sout = s + 1;
     }
@@ -349,8 +367,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT
         for (int a2 = 0; a2 < A2; ++a2) {
             const T ex = qa ? ea[it][a2] : eb[it][a2];
             const T ey = pb ? eb[it][a2] : ea[it][a2];
-            v[it][a2] = {q[it][a2].x * ex, q[it][a2].y * ey};
-            q[it][a2] = {q[it][a2].x * (ex * ex), q[it][a2].y * (ey * ey)};
+            v[it][a2] = {Boltz<T>::mul4(q[it][a2].x, ex), Boltz<T>::mul4(q[it][a2].y, ey)};
+            q[it][a2] = {Boltz<T>::mul2(q[it][a2].x, ex), Boltz<T>::mul2(q[it][a2].y, ey)};
         }
     __syncthreads();
     P::fwd(v, buf, twz, twy, a.Ch + fo, tid);
@@ -383,8 +401,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT
                 const int e = P::z1_elem(w, a2);
                 const T va = a.e4A[fo + e], vb = a.e4B[fo + e];
                 const T x4 = qa ? va : vb, y4 = pb ? vb : va;
-                ex[it][a2] = x4 * x4;
-                ey[it][a2] = y4 * y4;
+                ex[it][a2] = x4;
+                ey[it][a2] = y4;
             }
         }
     }
@@ -395,8 +413,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT
     for (int it = 0; it < IZ1; ++it)
 #pragma unroll
         for (int a2 = 0; a2 < A2; ++a2) {
-            z[it][a2].x *= ex[it][a2];
-            z[it][a2].y *= ey[it][a2];
+            z[it][a2].x = Boltz<T>::mul2(z[it][a2].x, ex[it][a2]);
+            z[it][a2].y = Boltz<T>::mul2(z[it][a2].y, ey[it][a2]);
         }
     __syncthreads();
     P::fwd(z, buf, twz, twy, a.Ch + fo, tid);
@@ -474,14 +492,10 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<N0, TC>::NT
             rdft<T, X1, false>(u[it]);
             // kinetic factor at kx = k2 + X2 k1 (exp(-k^2 ds/12)/M, or its square * M)
             const int y = (col < NC ? col : 0) / NZ, z = (col < NC ? col : 0) % NZ;
-            const T gyz = a.gy[y] * a.gz[z];
+            const T gyz = Kinetic<T>::col(a.gy[y], a.gz[z]);
             sfor<X1>([&](auto kc) {
                 constexpr int k1 = decltype(kc)::value;
-                T g = gx[k2 + X2 * k1] * gyz;
-                if (full) g = g * g;
-                g = g * a.invM;
-                u[it][k1].x *= g;
-                u[it][k1].y *= g;
+                Kinetic<T>::apply(u[it][k1], gx[k2 + X2 * k1], gyz, full, a.invM);
             });
             rdft<T, X1, true>(u[it]);
             sfor<X1>([&](auto nc) {

@@ -19,7 +19,7 @@ struct PropArgs {
     const cpx<T>* tw0;     // twiddle tables, lengths m0, m1, m2
     const cpx<T>* tw1;
     const cpx<T>* tw2;
-    const T* gx;           // exp(-kx^2 ds/12) per axis (k from setup_k_sq, density.h:79-95)
+    const T* gx;           // per-axis kinetic tables (see Kinetic<T>)
     const T* gy;
     const T* gz;
     T invM;
@@ -30,6 +30,88 @@ struct PropArgs {
 struct ChainConst {
     int N, n1, n2, Nh, ens;
     double alpha, phic, ds;
+};
+
+// Boltzmann-factor arithmetic per precision. a = (w - <w>) ds / 4 is the exponent of the
+// quarter-step factor E1/4 = exp(-a) (GPU_kernels.h:220); E1/2 = (E1/4)^2 (GPU_kernels.h:221).
+//
+// FP64 stores E1/4 itself and multiplies exactly as the reference does.
+// FP32 stores u = 1 - E1/4 = -expm1(-a) instead. |u| << 1, so rounding u to FP32 moves
+// the factor by ~|u| 2^-24 rather than 2^-24. Storing E1/4 in FP32 would bias every
+// point by the same relative error on each of the 4 uses per step, for all N steps. That
+// systematic drift is ~1e-5 in q(N) and in the free energy at N = 128. q*E1/4 is then
+// q - q*u, and q*E1/2 is q - q*v with v = u(2 - u), whose roundings vary with q and so
+// average out over the grid.
+template <typename T>
+struct Boltz;
+
+template <>
+struct Boltz<double> {
+    static __host__ __device__ __forceinline__ double from_arg(double a) { return exp(-a); }
+    static __host__ __device__ __forceinline__ double mul4(double q, double e) { return q * e; }
+    static __host__ __device__ __forceinline__ double mul2(double q, double e) { return q * (e * e); }
+    // (4 qs0 E1/4 - qs1 E1/2) / 3  (GPU_kernels.h:202)
+    static __host__ __device__ __forceinline__ double rich(double h0, double h1, double e)
+    {
+        return (h0 * e * 4.0 - h1 * (e * e)) / 3.0;
+    }
+};
+
+template <>
+struct Boltz<float> {
+    static __host__ __device__ __forceinline__ float from_arg(double a) { return (float)(-expm1(-a)); }
+    static __host__ __device__ __forceinline__ float mul4(float q, float u) { return q - q * u; }
+    static __host__ __device__ __forceinline__ float mul2(float q, float u)
+    {
+        const float v = u * (2.0f - u);
+        return q - q * v;
+    }
+    static __host__ __device__ __forceinline__ float rich(float h0, float h1, float u)
+    {
+        const float v = u * (2.0f - u);
+        return ((h0 - h0 * u) * 4.0f - (h1 - h1 * v)) / 3.0f;
+    }
+};
+
+// Kinetic-factor arithmetic per precision. With a_i = k_i^2 ds/12 per axis
+// (setup_k_sq, density.h:79-95), the half-step factor is exp(-(ax+ay+az))/M and the
+// full-step factor its square times M (GPU_kernels.h:207-213).
+//
+// FP64 tables hold g_i = exp(-a_i), so the factor is gx gy gz / M.
+// FP32 tables hold kappa_i = 1 - exp(-a_i) = -expm1(-a_i). The factor is applied as
+// (q - q kappa) / M, with kappa = 1 - prod(1 - kappa_i) assembled from sums of small
+// positive terms. A low-k mode decays by kappa ~ 1e-3 per step, so rounding the factor
+// itself to FP32 would misstate that decay by 2^-24 / kappa, identically on every step.
+template <typename T>
+struct Kinetic;
+
+template <>
+struct Kinetic<double> {
+    static double table(double a) { return exp(-a); }
+    // per-column part (y, z)
+    static __host__ __device__ __forceinline__ double col(double gy, double gz) { return gy * gz; }
+    static __host__ __device__ __forceinline__ void apply(cpx<double>& v, double gx, double gyz, bool full,
+                                                          double invM)
+    {
+        double g = gx * gyz;
+        if (full) g = g * g;
+        g = g * invM;
+        v.x *= g;
+        v.y *= g;
+    }
+};
+
+template <>
+struct Kinetic<float> {
+    static float table(double a) { return (float)(-expm1(-a)); }
+    static __host__ __device__ __forceinline__ float col(float ky, float kz) { return ky + kz - ky * kz; }
+    static __host__ __device__ __forceinline__ void apply(cpx<float>& v, float kx, float kyz, bool full, float invM)
+    {
+        float k = kx + kyz - kx * kyz;
+        if (full) k = k * (2.0f - k);
+        v.x = (v.x - v.x * k) * invM;
+        v.y = (v.y - v.y * k) * invM;
+    }
 };
 
 }  // namespace scft
