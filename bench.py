@@ -122,6 +122,29 @@ def precision_report(W, Wshape_m, c, oracle_rows=(0,)):
     return out
 
 
+def plain_fp64_rate(W, c, props_per_iter, reps=2):
+    """The plain per-replica FP64 GPU path (hipFFT, unfused, blocking host copies per props;
+    scft_string_amd/plain.py): time `reps` props of one replica and scale to the string
+    iteration's props count. Host-side field-update loops are not counted (generous to it)."""
+    import torch
+
+    from scft_string_amd.plain import PlainFP64
+
+    p = PlainFP64(c["m"], c["D"], c["N"])
+    Wr = W[1].reshape(2, -1)
+    p.props(Wr, c["N"], c["alpha"], c["phidb"], c["f"], c["ens"])  # warm-up (plans, first touch)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        p.props(Wr, c["N"], c["alpha"], c["phidb"], c["f"], c["ens"])
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    p.close()
+    return {"ms_per_props": dt * 1e3, "iters_per_s": 1.0 / (dt * props_per_iter),
+            "sample": f"{reps} props of one replica at {c['m'][0]}^3, N={c['N']}, scaled to {props_per_iter:.0f} "
+                      "props per string iteration"}
+
+
 def pmc_probe():
     """Workload for rocprofv3 --pmc runs: a 512 MiB device copy of known size (the scale for
     FETCH_SIZE / WRITE_SIZE) followed by one batched props of the benchmark string."""
@@ -153,6 +176,7 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--pmc-probe", action="store_true")
     a = ap.parse_args()
@@ -225,6 +249,10 @@ def main():
                                f"{n_act} replicas", "ms_per_contour_step": ms_step,
                      "algorithmic_bytes_per_contour_step": bstep},
     }
+    if not a.no_plain:
+        pl = plain_fp64_rate(Wend, c, props / a.steps)
+        pl["speedup_of_value"] = iters_per_s / pl["iters_per_s"]
+        out["plain_fp64_per_replica"] = pl
     if not a.no_accuracy:
         out["accuracy"] = precision_report(Wend, c["m"], c, oracle_rows=(0, R // 2))
     if not a.no_cpu:

@@ -53,6 +53,23 @@ def build_library(verbose=True):
     return LIB
 
 
+PLAIN_LIB = os.path.join(PKG, "libscft_plain.so")
+
+
+def build_plain(verbose=True):
+    """The benchmark's comparator: the plain per-replica FP64 path (hipFFT, unfused)."""
+    src = os.path.join(CSRC, "baseline", "plain_fp64.hip")
+    tmp = PLAIN_LIB + ".tmp"
+    cmd = [HIPCC, *COMMON, "-shared", src, "-o", tmp, "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed for plain_fp64.hip:\n" + r.stderr)
+    os.replace(tmp, PLAIN_LIB)
+    if verbose:
+        print("built", PLAIN_LIB)
+    return PLAIN_LIB
+
+
 def build_driver(verbose=True):
     src = os.path.join(CSRC, "driver", "scft_drive.cpp")
     cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", INC, "-I", os.path.join(CSRC, "driver"), src, "-o", DRIVER,
@@ -68,6 +85,7 @@ def main():
     ap.add_argument("--driver", action="store_true")
     a = ap.parse_args()
     build_library()
+    build_plain()
     if a.driver:
         build_driver()
 
