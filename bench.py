@@ -95,15 +95,24 @@ def precision_report(W, Wshape_m, c, oracle_rows=(0,)):
     R = W.shape[0]
     out = {}
     res = {}
+    fe = {}
     for prec in ("fp32", "fp64"):
         with Context(c["m"], c["D"], c["N"], c["f"], c["alpha"], c["ens"], c["phidb"], c["chi"], nrep=R,
                      precision=prec, fix=c["fix"]) as ctx:
             phi, lnQ, _ = ctx.props_batch(W)
+            ctx.set_fields(W)
+            fe[prec] = ctx.free_energy()["FEs"]  # FreeE: W+ relaxation + props (F in FP64)
         res[prec] = (phi, lnQ, free_energies(W, phi, lnQ, c["chi"]))
+    # FreeE of the FP32 context (FP32 W+ relaxation, F from an FP64 props) vs the FP64 context
+    dFE = np.abs(fe["fp32"] - fe["fp64"])
+    rel = (fe["fp32"] - fe["fp32"][0]) - (fe["fp64"] - fe["fp64"][0])
+    out["freeE_fp32_vs_fp64_max_abs_dF"] = float(dFE.max())
+    out["freeE_fp32_vs_fp64_max_abs_dF_between_replicas"] = float(np.abs(rel).max())
+    out["FEs_fp64"] = [float(v) for v in fe["fp64"]]
+    # one FP32 props against one FP64 props on identical fields (first order in FP32 roundoff)
     dF = np.abs(res["fp32"][2] - res["fp64"][2])
-    out["fp32_vs_fp64_max_abs_dF"] = float(dF.max())
-    out["fp32_vs_fp64_max_abs_dphiA"] = float(np.max(np.abs(res["fp32"][0][:, 0] - res["fp64"][0][:, 0])))
-    out["F_fp64"] = [float(v) for v in res["fp64"][2]]
+    out["props_fp32_vs_fp64_max_abs_dF"] = float(dF.max())
+    out["props_fp32_vs_fp64_max_abs_dphiA"] = float(np.max(np.abs(res["fp32"][0][:, 0] - res["fp64"][0][:, 0])))
     try:
         from oracle import orc
 

@@ -55,6 +55,8 @@ typedef struct scft_params {
     int justFE;      /* 1: only the free energy (main skips solve_field) */
     int springsteps; /* iterations with the spring term before redistribution starts */
     int device;      /* HIP device ordinal */
+    int fe_fp64;     /* FP32 contexts: 1 = the props that scft_free_energy evaluates F from
+                        runs in FP64 (on the FP32-relaxed fields), 0 = in FP32 */
 } scft_params;
 
 typedef struct scft_ctx scft_ctx;

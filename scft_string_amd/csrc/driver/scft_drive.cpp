@@ -84,6 +84,7 @@ Run set_parameters(const Params& P)
     p.justFE = 0;
     p.springsteps = 0;
     p.device = 0;
+    p.fe_fp64 = 1;
     auto get = [&](const char* k, double& dst) { if (has(P, k)) dst = P.at(k).front(); };
     auto geti = [&](const char* k, int& dst) { if (has(P, k)) dst = (int)P.at(k).front(); };
     get("chi", p.chi);
@@ -101,6 +102,7 @@ Run set_parameters(const Params& P)
     geti("maxiter", r.maxiter);
     geti("prerelax", r.prerelax);
     geti("device", p.device);
+    geti("fe_fp64", p.fe_fp64);
     if (has(P, "precision")) p.precision = P.at("precision").front() == 32 ? SCFT_FP32 : SCFT_FP64;
     if (has(P, "m")) for (size_t i = 0; i < P.at("m").size() && i < 3; ++i) p.m[i] = (int)P.at("m")[i];
     if (has(P, "D")) for (size_t i = 0; i < P.at("D").size() && i < 3; ++i) p.D[i] = P.at("D")[i];

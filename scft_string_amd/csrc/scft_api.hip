@@ -27,6 +27,8 @@ struct scft_ctx {
     std::string err;
     std::unique_ptr<PropEngine<float>> pe32;
     std::unique_ptr<PropEngine<double>> pe64;
+    // FP32 contexts: FP64 engine for the props that FreeE takes F from (created on first use)
+    std::unique_ptr<PropEngine<double>> pe64_fe;
     // field state
     double *dW = nullptr, *dphi = nullptr, *dDEV = nullptr, *ddWda = nullptr;
     // scratch rows for scft_props_batch
@@ -64,19 +66,22 @@ scft_status fail(scft_ctx* c, scft_status s, const std::string& msg)
         if (s_ != SCFT_OK) return s_;         \
     } while (0)
 
-// props for the slots listed in h_slots (replica indices into W/phi rows).
+// props for the slots listed in h_slots (replica indices into W/phi rows). fe64: use the
+// FP64 free-energy engine of an FP32 context.
 scft_status run_props(scft_ctx* c, const double* dW, long long wstride, double* dphi, long long phistride,
-                      const std::vector<int>& slots, double* lnQ_host, double* phis_host)
+                      const std::vector<int>& slots, double* lnQ_host, double* phis_host, bool fe64 = false)
 {
     const int nb = (int)slots.size();
     if (nb == 0) return SCFT_OK;
     for (int i = 0; i < nb; ++i) c->hslot[i] = slots[i];
     HIPCHK(c, hipMemcpyAsync(c->dslot, c->hslot, sizeof(int) * nb, hipMemcpyHostToDevice, c->st));
-    hipError_t e = c->pe32 ? c->pe32->run(dW, wstride, dphi, phistride, c->dslot, nb, c->cc, c->dphisv, c->st)
-                           : c->pe64->run(dW, wstride, dphi, phistride, c->dslot, nb, c->cc, c->dphisv, c->st);
+    PropEngine<double>* p64 = (fe64 && c->pe64_fe) ? c->pe64_fe.get() : c->pe64.get();
+    const bool use32 = c->pe32 && !(fe64 && c->pe64_fe);
+    hipError_t e = use32 ? c->pe32->run(dW, wstride, dphi, phistride, c->dslot, nb, c->cc, c->dphisv, c->st)
+                         : p64->run(dW, wstride, dphi, phistride, c->dslot, nb, c->cc, c->dphisv, c->st);
     if (e != hipSuccess) return fail(c, SCFT_ERR_HIP, std::string("propagator launch: ") + hipGetErrorString(e));
     if (lnQ_host || phis_host) {
-        const double* scal = c->pe32 ? c->pe32->scal() : c->pe64->scal();
+        const double* scal = use32 ? c->pe32->scal() : p64->scal();
         HIPCHK(c, hipMemcpyAsync(c->hsm, scal, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, c->st));
         std::vector<double> ph(5 * (size_t)nb);
         if (phis_host)
@@ -230,6 +235,7 @@ scft_status scft_ctx_destroy(scft_ctx* c)
     if (c->st) (void)hipStreamSynchronize(c->st);
     c->pe32.reset();
     c->pe64.reset();
+    c->pe64_fe.reset();
     void* dptrs[] = {c->dW, c->dphi, c->dDEV, c->ddWda, c->dWs, c->dphis_rows, c->dpart, c->dsm, c->dlam,
                      c->dx, c->dxref, c->dcproj, c->dphisv, c->dactive, c->dslot};
     for (void* q : dptrs)
@@ -511,7 +517,19 @@ scft_status scft_free_energy(scft_ctx* c, double* FE, double* FEs, double* FEs0,
     SCFTCHK(scft_solve_w_plus(c, 100, 1e-4, which.data(), nullptr));
     const int nb = (int)set.size();
     std::vector<double> lq(nb);
-    SCFTCHK(run_props(c, c->dW, 2 * M, c->dphi, 4 * M, set, lq.data(), nullptr));
+    // In an FP32 context the props that F is taken from runs in FP64: F is stationary in the
+    // fields at a converged string point, so evaluating it in FP64 on the FP32-relaxed fields
+    // leaves only a second-order error from the FP32 relaxation.
+    const bool fe64 = c->pe32 && c->p.fe_fp64;
+    if (fe64 && !c->pe64_fe) {
+        c->pe64_fe = std::make_unique<PropEngine<double>>();
+        const std::string msg = c->pe64_fe->init(c->p.m, c->p.D, c->p.N, c->R);
+        if (!msg.empty()) {
+            c->pe64_fe.reset();
+            return fail(c, SCFT_ERR_NOMEM, "FP64 free-energy engine: " + msg);
+        }
+    }
+    SCFTCHK(run_props(c, c->dW, 2 * M, c->dphi, 4 * M, set, lq.data(), nullptr, fe64));
     scft::launch_fe_sums(c->dslot, nb, M, c->p.chi, c->dW, c->dphi, c->dpart, c->nblk, c->st);
     HIPCHK(c, hipGetLastError());
     std::vector<double> fs;

@@ -1,0 +1,67 @@
+"""Mixed precision: an FP32 context evaluates FreeE's free energy from an FP64 props on its
+FP32-relaxed fields, so its F agrees with an FP64 context's to second order in the FP32
+field error (bar: |dF| <= 1e-6 per replica and between replicas).
+
+That argument needs F to be stationary in W+ at the W+ saddle. It holds in the canonical
+ensemble when n1, n2 and Nh are even, because the Simpson weights then integrate exactly.
+With an odd count the reference's weights make the copolymer normalisation differ from
+phidb (density.h:405-422), so F drifts with the uniform part of W+, which canonical
+relaxation never removes. In the grand-canonical ensemble the reference's F uses
+Q2 + phidb*QC for lnQ (density.h:398), which is stationary only up to that same
+discretisation, so its bar is 1e-5."""
+import numpy as np
+import pytest
+
+from scft_string_amd import Context
+
+pytestmark = pytest.mark.gpu
+
+
+def _string(m, D, R, seed=3):
+    M = int(np.prod(m))
+    rng = np.random.default_rng(seed)
+    x = (np.arange(m[0]) * D[0] / m[0])[:, None, None] * np.ones(m)
+    y = (np.arange(m[1]) * D[1] / m[1])[None, :, None] * np.ones(m)
+    lam = 0.5 * np.cos(2 * np.pi * x / D[0])
+    other = 0.5 * np.cos(2 * np.pi * x / D[0]) * np.cos(2 * np.pi * y / D[1])
+    W = np.zeros((R, 2, M))
+    for i in range(R):
+        t = i / (R - 1)
+        W[i, 0] = ((1 - t) * lam + t * other).reshape(-1) + 0.01 * rng.standard_normal(M)
+    return W
+
+
+@pytest.mark.parametrize("ens,f,bar", [(1, 0.5, 1e-6), (2, 0.45, 1e-5)])
+def test_free_energy_fp32_context_matches_fp64(ens, f, bar):
+    m, D, N, R = (32, 32, 32), (4.0, 4.0, 4.0), 64, 4
+    phidb = 0.85 if ens == 1 else 0.5
+    W = _string(m, D, R)
+    fe = {}
+    first_order = {}
+    # evolve the string a little in FP32, then relax W+ of every replica (ends included)
+    # tightly, so FreeE's own W+ sub-iterations start at the W+ saddle in both precisions
+    with Context(m, D, N, f, 0.125, ens, phidb, 16.0, nrep=R, precision="fp32", fix=(1, 1)) as ctx:
+        ctx.set_fields(W)
+        for k in range(2):
+            ctx.string_iterate(k)
+        Wk = ctx.get_fields()
+    with Context(m, D, N, f, 0.125, ens, phidb, 16.0, nrep=R, precision="fp64", fix=(1, 1)) as ctx:
+        ctx.set_fields(Wk)
+        ctx.solve_w_plus(3000, 1e-7)
+        Wk = ctx.get_fields()
+    # F of the same fields in both precisions
+    for prec in ("fp32", "fp64"):
+        with Context(m, D, N, f, 0.125, ens, phidb, 16.0, nrep=R, precision=prec, fix=(1, 1)) as ctx:
+            ctx.set_fields(Wk)
+            fe[prec] = ctx.free_energy()["FEs"]
+        with Context(m, D, N, f, 0.125, ens, phidb, 16.0, nrep=R, precision=prec, fix=(1, 1),
+                     fe_fp64=0) as ctx:
+            ctx.set_fields(Wk)
+            first_order[prec] = ctx.free_energy()["FEs"]
+    d = np.abs(fe["fp32"] - fe["fp64"])
+    between = np.abs((fe["fp32"] - fe["fp32"][0]) - (fe["fp64"] - fe["fp64"][0]))
+    d1 = np.abs(first_order["fp32"] - first_order["fp64"])
+    print(f"ens={ens}: FreeE |dF| max {d.max():.2e}, between replicas {between.max():.2e}; "
+          f"all-FP32 FreeE |dF| max {d1.max():.2e}", flush=True)
+    assert d.max() <= bar
+    assert between.max() <= bar
