@@ -68,6 +68,10 @@ scft_status scft_ctx_create(const scft_params *p, scft_ctx **out);
 scft_status scft_ctx_destroy(scft_ctx *ctx);
 const char *scft_last_error(const scft_ctx *ctx);
 scft_status scft_sync(scft_ctx *ctx);
+/* FFT kernels the propagator runs on: "two-stage" or "stockham" (plane/column kernels
+ * compiled for the grid) or "pencil" (one line pass per axis, any 2-3-5 grid). The
+ * reference has no counterpart: cuFFT picks its own plan (density.h:41-75). */
+const char *scft_fft_path(const scft_ctx *ctx);
 
 /* Replaces density::props (density.h:344-459), batched over `nrep` independent rows:
  * W [nrep][2M] in; phi [nrep][4M], lnQ [nrep], phis [nrep][5] out (phi, phis may be NULL).

@@ -18,8 +18,9 @@ EXPORTED = [
     "scft_ctx_create", "scft_ctx_destroy", "scft_last_error", "scft_sync", "scft_props_batch",
     "scft_props_batch_device", "scft_set_fields", "scft_get_fields", "scft_get_densities",
     "scft_get_tangent", "scft_solve_w_plus", "scft_string_iterate", "scft_free_energy",
-    "scft_debug_qq_invariant", "scft_debug_time_props",
+    "scft_debug_qq_invariant", "scft_debug_time_props", "scft_fft_path",
 ]
+_STRING_RESULT = ("scft_last_error", "scft_fft_path")
 
 
 class ScftError(RuntimeError):
@@ -72,8 +73,10 @@ def lib():
         L.scft_free_energy.argtypes = [vp, dp, dp, dp, dp, dp, dp]
         L.scft_debug_qq_invariant.argtypes = [vp, C.c_int, dp]
         L.scft_debug_time_props.argtypes = [vp, C.c_int, C.c_int, dp]
+        L.scft_fft_path.argtypes = [vp]
+        L.scft_fft_path.restype = C.c_char_p
         for name in EXPORTED:
-            if name not in ("scft_last_error",):
+            if name not in _STRING_RESULT:
                 getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
@@ -125,6 +128,11 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+    @property
+    def fft_path(self):
+        """"two-stage", "stockham" or "pencil": the propagator's FFT kernels for this grid."""
+        return lib().scft_fft_path(self._h).decode()
 
     # ---------------------------------------------------------------- props (density.h:344)
     def props_batch(self, W):

@@ -2,6 +2,7 @@
 #include "prop_engine.hpp"
 #include "prop_kernels.hpp"
 #include "prop_kernels2.hpp"
+#include "prop_pencil.hpp"
 
 #include <cmath>
 #include <cstdio>
@@ -141,9 +142,72 @@ const KernelSet<double>* find_kernels<double>(int m0, int m1, int m2)
 
 std::string supported_grids(bool fp64)
 {
-    std::string s = "8^3 12^3 16^3 24^3 32^3 48^3 64^3 96^3 12x8x16 8x12x10 16x12x20";
+    std::string s = "plane/column kernels for 8^3 12^3 16^3 24^3 32^3 48^3 64^3 96^3 12x8x16 8x12x10 16x12x20";
     if (!fp64) s += " 128^3";
+    s += "; line kernels for any grid whose axes are 1.." + std::to_string(PC_LMAX) + " and factor into 2, 3, 5";
     return s;
+}
+
+// ---------------------------------------------------------------- pencil (line) kernels
+template <typename T>
+static hipError_t pencil_setup(int m0, int m1, int m2)
+{
+    const int lds = (int)pencil_lds_bytes<T>(m0, m1, m2);
+    const void* fns[] = {(const void*)k_pz<T>, (const void*)k_py<T, false>, (const void*)k_py<T, true>,
+                         (const void*)k_px<T>};
+    for (const void* f : fns) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+template <typename T>
+static void pencil_pz(const PropArgs<T>& a, int s, int mode, const PencilTiles& p, size_t lds, hipStream_t st)
+{
+    const long long lines = (long long)a.m0 * a.m1;
+    hipLaunchKernelGGL((k_pz<T>), dim3((unsigned)((lines + p.nlz - 1) / p.nlz), a.nb), dim3(PC_NT), lds, st, a, s,
+                       mode, p.nlz);
+}
+
+// y pass over nfields fields (the first nh are Ch)
+template <typename T, bool INV>
+static void pencil_py(const PropArgs<T>& a, int nh, int nfields, const PencilTiles& p, size_t lds, hipStream_t st)
+{
+    const int ntz = (a.m2 + p.ty - 1) / p.ty;
+    hipLaunchKernelGGL((k_py<T, INV>), dim3(a.m0 * ntz, nfields), dim3(PC_NT), lds, st, a, nh, p.ty);
+}
+
+template <typename T>
+static void pencil_px(const PropArgs<T>& a, int nh, int nfields, const PencilTiles& p, size_t lds, hipStream_t st)
+{
+    const int ntz = (a.m2 + p.tx - 1) / p.tx;
+    hipLaunchKernelGGL((k_px<T>), dim3(a.m1 * ntz, nfields), dim3(PC_NT), lds, st, a, nh, p.tx);
+}
+
+template <typename T>
+static void pencil_begin(const PropArgs<T>& a, hipStream_t st)
+{
+    const PencilTiles p = pencil_tiles(a.m0, a.m1, a.m2);
+    const size_t lds = pencil_lds_bytes<T>(a.m0, a.m1, a.m2);
+    pencil_pz<T>(a, 0, PZ_BEGIN, p, lds, st);
+    pencil_py<T, false>(a, a.nb, 2 * a.nb, p, lds, st);
+}
+
+template <typename T>
+static void pencil_step(const PropArgs<T>& a, int s, hipStream_t st)
+{
+    const PencilTiles p = pencil_tiles(a.m0, a.m1, a.m2);
+    const size_t lds = pencil_lds_bytes<T>(a.m0, a.m1, a.m2);
+    const bool last = s == a.N - 1;
+    pencil_px<T>(a, a.nb, 2 * a.nb, p, lds, st);
+    pencil_py<T, true>(a, a.nb, a.nb, p, lds, st);
+    pencil_pz<T>(a, s, PZ_MID, p, lds, st);
+    pencil_py<T, false>(a, a.nb, a.nb, p, lds, st);
+    pencil_px<T>(a, a.nb, a.nb, p, lds, st);
+    pencil_py<T, true>(a, a.nb, 2 * a.nb, p, lds, st);
+    pencil_pz<T>(a, s, last ? PZ_LAST : PZ_STEP, p, lds, st);
+    if (!last) pencil_py<T, false>(a, a.nb, 2 * a.nb, p, lds, st);
 }
 
 // ---------------------------------------------------------------- PropEngine
@@ -205,7 +269,11 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
     N_ = N;
     nbmax_ = nbmax;
     M_ = (long long)m[0] * m[1] * m[2];
-    ks_ = find_kernels<T>(m[0], m[1], m[2]);
+    // Plane/column kernels where this grid has them; the line kernels otherwise, or when
+    // SCFT_PENCIL=1 is set.
+    const char* pen = getenv("SCFT_PENCIL");
+    ks_ = (pen && pen[0] == '1') ? nullptr : find_kernels<T>(m[0], m[1], m[2]);
+    hipError_t e = hipSuccess;
     if (ks_) {
         const char* legacy = getenv("SCFT_LEGACY_FFT");
         const bool use_two = ks_->step2 && !(legacy && legacy[0] == '1');
@@ -213,14 +281,18 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
         begin_ = use_two ? ks_->begin2 : ks_->begin;
         step_ = use_two ? ks_->step2 : ks_->step;
         path_ = use_two ? "two-stage" : "stockham";
-    }
-    if (!ks_) {
-        char b[256];
+        e = setup_();
+    } else if (pencil_supported(m[0], m[1], m[2])) {
+        begin_ = &pencil_begin<T>;
+        step_ = &pencil_step<T>;
+        path_ = "pencil";
+        e = pencil_setup<T>(m[0], m[1], m[2]);
+    } else {
+        char b[512];
         snprintf(b, sizeof b, "no %s propagator kernels for grid %dx%dx%d (supported: %s)",
                  sizeof(T) == 8 ? "FP64" : "FP32", m[0], m[1], m[2], supported_grids(sizeof(T) == 8).c_str());
         return b;
     }
-    hipError_t e = setup_();
     if (e != hipSuccess) return std::string("hipFuncSetAttribute: ") + hipGetErrorString(e);
     nblk_ = (int)std::min<long long>(1024, (M_ + RED_NT - 1) / RED_NT);
     auto alloc = [&](void** p, size_t bytes) -> bool {
@@ -286,6 +358,9 @@ hipError_t PropEngine<T>::run(const double* dW, long long wstride, double* dphi,
     if (nb <= 0) return hipSuccess;
     PropArgs<T> a;
     a.M = M_;
+    a.m0 = m_[0];
+    a.m1 = m_[1];
+    a.m2 = m_[2];
     a.N = cc.N;
     a.n1 = cc.n1;
     a.n2 = cc.n2;

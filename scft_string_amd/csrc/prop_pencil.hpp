@@ -1,0 +1,305 @@
+// prop_pencil.hpp -- propagator kernels for any grid whose axes factor into 2, 3, 5: one
+// 1-D line pass per axis, with lengths known only at run time.
+//
+// The plane kernels need a whole (y, z) plane in LDS. For the reference's shipped box
+// (m = 200 x 150 x 150, input.dat) a plane is 180 KB in FP32 and 360 KB in FP64, so it
+// does not fit in 160 KB. Here every pass holds a tile of whole lines instead:
+//   k_pz  z lines (contiguous). Inverse z -> real-space Boltzmann / Richardson work -> forward z
+//   k_py  y lines of one x and a tile of z (stride m2), forward or inverse
+//   k_px  x lines of one y and a tile of z (stride m1 m2): forward, kinetic factor, inverse
+// The field layout and the packed z = q + i q-dagger convention are those of the plane
+// kernels, so the q store, the Simpson densities and every reduction are shared. A contour
+// step s (density.h:129-220) is
+//   px(Ch, Cf) -> py^-1(Ch) -> pz_mid -> py(Ch) -> px(Ch) -> py^-1(Ch, Cf) -> pz_end -> py(Ch, Cf)
+// where between launches Ch and Cf hold fields transformed along y and z but not x.
+//
+// Line transforms are Stockham autosort passes (radix 8, 4, 2, 3, 5, chosen at run time by
+// pick_radix) over lines stored element-major in LDS: element e of line l sits at
+// buf[e * P + l] with an odd pitch P, so consecutive lanes take consecutive lines.
+#pragma once
+#include "prop_types.hpp"
+
+namespace scft {
+
+constexpr int PC_NT = 256;     // threads per workgroup of every pencil kernel
+constexpr int PC_EMAX = 2048;  // complex elements (lines x length) per workgroup transform
+constexpr int PC_LMAX = 1024;  // longest axis
+
+enum PencilMode { PZ_BEGIN = 0, PZ_STEP = 1, PZ_LAST = 2, PZ_MID = 3 };
+
+__host__ __device__ constexpr int pc_bpt(int R) { return (PC_EMAX / R + PC_NT - 1) / PC_NT; }
+__host__ __device__ constexpr int pc_pitch(int nlines) { return nlines + ((nlines & 1) ? 0 : 1); }
+
+// One Stockham pass of radix R on nl lines of length L, NS = product of earlier radices.
+// All reads to registers, barrier, all writes, barrier. Needs nl * L <= PC_EMAX.
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void pc_pass(cpx<T>* buf, int L, int NS, int nl, int P, const cpx<T>* tw, int tid)
+{
+    constexpr int BPT = pc_bpt(R);
+    const int NB = L / R, total = nl * NB, tws = L / (NS * R);
+    cpx<T> v[BPT][R];
+#pragma unroll
+    for (int it = 0; it < BPT; ++it) {
+        const int w = tid + it * PC_NT;
+        if (w < total) {
+            const int line = w % nl, j = w / nl;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[it][r] = buf[(j + r * NB) * P + line];
+            if (NS > 1) {
+                const int k = j % NS;
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const cpx<T> t = tw[r * k * tws];
+                    v[it][r] = INV ? cmulc(v[it][r], t) : cmul(v[it][r], t);
+                }
+            }
+            dft<R, INV, T>::run(v[it]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < BPT; ++it) {
+        const int w = tid + it * PC_NT;
+        if (w < total) {
+            const int line = w % nl, j = w / nl;
+            const int k = j % NS;
+            const int idx = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[(idx + r * NS) * P + line] = v[it][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Unnormalised DFT of nl lines of length L in place (forward sign -1, inverse +1). The
+// caller has synchronised after filling buf; the transform ends with a barrier.
+template <typename T, bool INV>
+__device__ void pc_fft(cpx<T>* buf, int L, int nl, int P, const cpx<T>* tw, int tid)
+{
+    for (int NS = 1; NS < L;) {
+        const int R = pick_radix(L / NS);
+        switch (R) {
+            case 8: pc_pass<T, 8, INV>(buf, L, NS, nl, P, tw, tid); break;
+            case 4: pc_pass<T, 4, INV>(buf, L, NS, nl, P, tw, tid); break;
+            case 2: pc_pass<T, 2, INV>(buf, L, NS, nl, P, tw, tid); break;
+            case 3: pc_pass<T, 3, INV>(buf, L, NS, nl, P, tw, tid); break;
+            default: pc_pass<T, 5, INV>(buf, L, NS, nl, P, tw, tid); break;
+        }
+        NS *= R;
+    }
+}
+
+// z-line pass. grid (ceil(m0 m1 / nlz), nb slots), block PC_NT.
+//   PZ_MID:   Ch <- Fz(E1/2 Fz^-1(Ch))                         (density.h:156-173)
+//   PZ_STEP:  q(s+1) = (4 Fz^-1(Ch) E1/4 - Fz^-1(Cf) E1/2)/3   (GPU_kernels.h:199-203), stored;
+//             then Ch <- Fz(E1/4 q), Cf <- Fz(E1/2 q) with step s+1's species
+//   PZ_LAST:  as PZ_STEP without the forward part
+//   PZ_BEGIN: q(0) = 1 stored, Ch <- Fz(E1/4), Cf <- Fz(E1/2) for step 0 (density.h:351-355)
+// LDS: lines [0, nlz) hold Ch, [nlz, 2 nlz) hold Cf, pitch pc_pitch(2 nlz), then m2 twiddles.
+template <typename T>
+__global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, int nlz)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int m2 = a.m2;
+    const int nfl = mode == PZ_MID ? 1 : 2;
+    const int nlines = nfl * nlz, P = pc_pitch(nlines);
+    cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* tw = buf + (size_t)m2 * P;
+    const int tid = threadIdx.x, slot = blockIdx.y;
+    const long long l0 = (long long)blockIdx.x * nlz;
+    const long long lrem = (long long)a.m0 * a.m1 - l0;
+    const int nl = lrem < nlz ? (int)lrem : nlz;  // valid lines of this workgroup
+    const size_t fo = (size_t)slot * (size_t)a.M + (size_t)l0 * m2;
+    const int ne = nlz * m2;
+    for (int i = tid; i < m2; i += PC_NT) tw[i] = a.tw2[i];
+
+    if (mode != PZ_BEGIN) {
+        for (int e = tid; e < ne; e += PC_NT) {
+            const int l = e / m2, z = e - l * m2;
+            const bool ok = l < nl;
+            buf[z * P + l] = ok ? a.Ch[fo + e] : cpx<T>{T(0), T(0)};
+            if (nfl == 2) buf[z * P + nlz + l] = ok ? a.Cf[fo + e] : cpx<T>{T(0), T(0)};
+        }
+        __syncthreads();
+        pc_fft<T, true>(buf, m2, nlines, P, tw, tid);
+    }
+
+    if (mode == PZ_MID) {
+        const bool qa = s < a.n1, pb = s < a.n2;  // species of this step (density.h:136-146)
+        for (int e = tid; e < ne; e += PC_NT) {
+            const int l = e / m2, z = e - l * m2;
+            if (l < nl) {
+                const T va = a.e4A[fo + e], vb = a.e4B[fo + e];
+                cpx<T>& v = buf[z * P + l];
+                v.x = Boltz<T>::mul2(v.x, qa ? va : vb);
+                v.y = Boltz<T>::mul2(v.y, pb ? vb : va);
+            }
+        }
+        __syncthreads();
+        pc_fft<T, false>(buf, m2, nlines, P, tw, tid);
+        for (int e = tid; e < ne; e += PC_NT) {
+            const int l = e / m2, z = e - l * m2;
+            if (l < nl) a.Ch[fo + e] = buf[z * P + l];
+        }
+        return;
+    }
+
+    const int sout = mode == PZ_BEGIN ? 0 : s + 1;
+    cpx<T>* qo = a.qs + ((size_t)slot * (size_t)(a.N + 1) + (size_t)sout) * (size_t)a.M + (size_t)l0 * m2;
+    const bool qa = s < a.n1, pb = s < a.n2;        // this step's species
+    const bool na = sout < a.n1, nbb = sout < a.n2;  // next step's species
+    for (int e = tid; e < ne; e += PC_NT) {
+        const int l = e / m2, z = e - l * m2;
+        if (l < nl) {
+            const T va = a.e4A[fo + e], vb = a.e4B[fo + e];
+            cpx<T> q;
+            if (mode == PZ_BEGIN) {
+                q = {T(1), T(1)};
+            } else {
+                const cpx<T> h0 = buf[z * P + l], h1 = buf[z * P + nlz + l];
+                q.x = Boltz<T>::rich(h0.x, h1.x, qa ? va : vb);
+                q.y = Boltz<T>::rich(h0.y, h1.y, pb ? vb : va);
+            }
+            qo[e] = q;
+            if (mode != PZ_LAST) {
+                const T ex = na ? va : vb, ey = nbb ? vb : va;
+                buf[z * P + l] = {Boltz<T>::mul4(q.x, ex), Boltz<T>::mul4(q.y, ey)};
+                buf[z * P + nlz + l] = {Boltz<T>::mul2(q.x, ex), Boltz<T>::mul2(q.y, ey)};
+            }
+        } else if (mode == PZ_BEGIN) {
+            buf[z * P + l] = {T(0), T(0)};
+            buf[z * P + nlz + l] = {T(0), T(0)};
+        }
+    }
+    if (mode == PZ_LAST) return;
+    __syncthreads();
+    pc_fft<T, false>(buf, m2, nlines, P, tw, tid);
+    for (int e = tid; e < ne; e += PC_NT) {
+        const int l = e / m2, z = e - l * m2;
+        if (l < nl) {
+            a.Ch[fo + e] = buf[z * P + l];
+            a.Cf[fo + e] = buf[z * P + nlz + l];
+        }
+    }
+}
+
+// Field f of a pencil launch: f < nh is Ch of slot f, else Cf of slot f - nh.
+template <typename T>
+__device__ __forceinline__ cpx<T>* pc_field(const PropArgs<T>& a, int f, int nh)
+{
+    return f < nh ? a.Ch + (size_t)f * (size_t)a.M : a.Cf + (size_t)(f - nh) * (size_t)a.M;
+}
+
+// y-line pass, forward or inverse, on tz z-columns of one x.
+// grid (m0 * ceil(m2 / tz), nh + nf fields), block PC_NT. LDS: m1 x pc_pitch(tz), m1 twiddles.
+template <typename T, bool INV>
+__global__ void __launch_bounds__(PC_NT) k_py(PropArgs<T> a, int nh, int tz)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int m1 = a.m1, m2 = a.m2, P = pc_pitch(tz);
+    cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* tw = buf + (size_t)m1 * P;
+    const int tid = threadIdx.x;
+    const int ntz = (m2 + tz - 1) / tz;
+    const int x = blockIdx.x / ntz, z0 = (blockIdx.x - x * ntz) * tz;
+    const int nz = m2 - z0 < tz ? m2 - z0 : tz;
+    cpx<T>* g = pc_field(a, blockIdx.y, nh) + (size_t)x * m1 * m2 + z0;
+    for (int i = tid; i < m1; i += PC_NT) tw[i] = a.tw1[i];
+    const int ne = m1 * tz;
+    for (int e = tid; e < ne; e += PC_NT) {
+        const int y = e / tz, zz = e - y * tz;
+        buf[y * P + zz] = zz < nz ? g[(size_t)y * m2 + zz] : cpx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    pc_fft<T, INV>(buf, m1, tz, P, tw, tid);
+    for (int e = tid; e < ne; e += PC_NT) {
+        const int y = e / tz, zz = e - y * tz;
+        if (zz < nz) g[(size_t)y * m2 + zz] = buf[y * P + zz];
+    }
+}
+
+// x-line pass on tx z-columns of one y: forward x, times exp(-k^2 ds/12)/M (Ch) or
+// exp(-k^2 ds/6)/M (Cf), inverse x (density.h:149-156, 198-205; GPU_kernels.h:175-181, 207-213).
+// grid (m1 * ceil(m2 / tx), nh + nf fields), block PC_NT. LDS: m0 x pc_pitch(tx), m0 twiddles.
+template <typename T>
+__global__ void __launch_bounds__(PC_NT) k_px(PropArgs<T> a, int nh, int tx)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int m0 = a.m0, m2 = a.m2, P = pc_pitch(tx);
+    const size_t sx = (size_t)a.m1 * m2;
+    cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* tw = buf + (size_t)m0 * P;
+    const int tid = threadIdx.x;
+    const int ntz = (m2 + tx - 1) / tx;
+    const int y = blockIdx.x / ntz, z0 = (blockIdx.x - y * ntz) * tx;
+    const int nz = m2 - z0 < tx ? m2 - z0 : tx;
+    const bool full = (int)blockIdx.y >= nh;
+    cpx<T>* g = pc_field(a, blockIdx.y, nh) + (size_t)y * m2 + z0;
+    for (int i = tid; i < m0; i += PC_NT) tw[i] = a.tw0[i];
+    const int ne = m0 * tx;
+    for (int e = tid; e < ne; e += PC_NT) {
+        const int x = e / tx, zz = e - x * tx;
+        buf[x * P + zz] = zz < nz ? g[x * sx + zz] : cpx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    pc_fft<T, false>(buf, m0, tx, P, tw, tid);
+    const T gy = a.gy[y];
+    for (int e = tid; e < ne; e += PC_NT) {
+        const int x = e / tx, zz = e - x * tx;
+        if (zz < nz) Kinetic<T>::apply(buf[x * P + zz], a.gx[x], Kinetic<T>::col(gy, a.gz[z0 + zz]), full, a.invM);
+    }
+    __syncthreads();
+    pc_fft<T, true>(buf, m0, tx, P, tw, tid);
+    for (int e = tid; e < ne; e += PC_NT) {
+        const int x = e / tx, zz = e - x * tx;
+        if (zz < nz) g[x * sx + zz] = buf[x * P + zz];
+    }
+}
+
+// ---------------------------------------------------------------- host-side tiling
+struct PencilTiles {
+    int nlz;  // z lines per k_pz workgroup (two fields each)
+    int ty;   // z columns per k_py workgroup
+    int tx;   // z columns per k_px workgroup
+};
+
+// Tiling for an m0 x m1 x m2 grid; every transform stays within PC_EMAX elements.
+inline PencilTiles pencil_tiles(int m0, int m1, int m2)
+{
+    auto tile = [](int len, int cap) {
+        int t = PC_EMAX / len;
+        if (t > cap) t = cap;
+        if (t < 1) t = 1;
+        const int n = (cap + t - 1) / t;  // tiles along z, then the smallest even split
+        return (cap + n - 1) / n;
+    };
+    PencilTiles p;
+    p.nlz = PC_EMAX / 2 / m2;
+    if (p.nlz < 1) p.nlz = 1;
+    p.ty = tile(m1, m2);
+    p.tx = tile(m0, m2);
+    return p;
+}
+
+inline bool pencil_supported(int m0, int m1, int m2)
+{
+    const int m[3] = {m0, m1, m2};
+    for (int i = 0; i < 3; ++i)
+        if (m[i] < 1 || m[i] > PC_LMAX || !fft_len_supported(m[i])) return false;
+    return true;
+}
+
+// dynamic LDS bytes of the three kernels (the largest is what setup reserves)
+template <typename T>
+inline size_t pencil_lds_bytes(int m0, int m1, int m2)
+{
+    const PencilTiles p = pencil_tiles(m0, m1, m2);
+    const size_t z = (size_t)m2 * pc_pitch(2 * p.nlz) + m2;
+    const size_t y = (size_t)m1 * pc_pitch(p.ty) + m1;
+    const size_t x = (size_t)m0 * pc_pitch(p.tx) + m0;
+    size_t b = z > y ? z : y;
+    b = b > x ? b : x;
+    return b * sizeof(cpx<T>);
+}
+
+}  // namespace scft

@@ -9,12 +9,13 @@ constexpr int RED_NT = 256;  // threads of the reduction / pointwise kernels
 template <typename T>
 struct PropArgs {
     long long M;           // grid points
+    int m0, m1, m2;        // grid (x slowest, z fastest)
     int N, n1, n2, Nh;     // contour steps, block split, homopolymer length
     int nb;                // active batch slots
     cpx<T>* Ch;            // [slot][M] half-step sandwich work field
     cpx<T>* Cf;            // [slot][M] full-step sandwich work field
-    const T* e4A;          // [slot][M] exp(-(wA - cA) ds / 4)
-    const T* e4B;          // [slot][M] exp(-(wB - cB) ds / 4)
+    const T* e4A;          // [slot][M] quarter-step factor of species A (Boltz<T> form)
+    const T* e4B;          // [slot][M] the same for species B
     cpx<T>* qs;            // [slot][N+1][M] stored (q1, q2)
     const cpx<T>* tw0;     // twiddle tables, lengths m0, m1, m2
     const cpx<T>* tw1;

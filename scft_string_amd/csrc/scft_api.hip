@@ -150,6 +150,13 @@ extern "C" {
 
 const char* scft_last_error(const scft_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+const char* scft_fft_path(const scft_ctx* ctx)
+{
+    if (!ctx) return "";
+    if (ctx->pe32) return ctx->pe32->fft_path();
+    return ctx->pe64 ? ctx->pe64->fft_path() : "";
+}
+
 scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
 {
     if (!p || !out) return SCFT_ERR_INVALID;
