@@ -21,6 +21,9 @@
 
 namespace scft {
 
+// Tile size: measured on MI355X against 512 x 4096, 256 x 1024 and 128 x 1024 (threads x
+// elements). Larger tiles serialise more work behind each barrier; smaller ones cut
+// the contiguous run of the strided passes.
 constexpr int PC_NT = 256;     // threads per workgroup of every pencil kernel
 constexpr int PC_EMAX = 2048;  // complex elements (lines x length) per workgroup transform
 constexpr int PC_LMAX = 1024;  // longest axis
@@ -89,6 +92,12 @@ __device__ void pc_fft(cpx<T>* buf, int L, int nl, int P, const cpx<T>* tw, int 
     }
 }
 
+// Element e of a workgroup's tile is handled by thread e % PC_NT in slot e / PC_NT of
+// its registers. Every global load of a kernel goes to registers first, in fully unrolled
+// loops, so all of a thread's loads are in flight together before the first LDS write.
+constexpr int PZ_EPT = PC_EMAX / 2 / PC_NT;  // per field in k_pz (nlz m2 <= PC_EMAX / 2)
+constexpr int PC_EPT = PC_EMAX / PC_NT;      // in k_py, k_px (lines x length <= PC_EMAX)
+
 // z-line pass. grid (ceil(m0 m1 / nlz), nb slots), block PC_NT.
 //   PZ_MID:   Ch <- Fz(E1/2 Fz^-1(Ch))                         (density.h:156-173)
 //   PZ_STEP:  q(s+1) = (4 Fz^-1(Ch) E1/4 - Fz^-1(Cf) E1/2)/3   (GPU_kernels.h:199-203), stored;
@@ -110,15 +119,30 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
     const long long lrem = (long long)a.m0 * a.m1 - l0;
     const int nl = lrem < nlz ? (int)lrem : nlz;  // valid lines of this workgroup
     const size_t fo = (size_t)slot * (size_t)a.M + (size_t)l0 * m2;
-    const int ne = nlz * m2;
+    const int ne = nl * m2, nt = nlz * m2;  // valid elements, tile elements
+
+    T va[PZ_EPT], vb[PZ_EPT];
+    cpx<T> c0[PZ_EPT], c1[PZ_EPT];
+#pragma unroll
+    for (int i = 0; i < PZ_EPT; ++i) {
+        const int e = tid + i * PC_NT;
+        const bool ok = e < ne;
+        va[i] = ok ? a.e4A[fo + e] : T(0);
+        vb[i] = ok ? a.e4B[fo + e] : T(0);
+        c0[i] = (ok && mode != PZ_BEGIN) ? a.Ch[fo + e] : cpx<T>{T(0), T(0)};
+        c1[i] = (ok && nfl == 2 && mode != PZ_BEGIN) ? a.Cf[fo + e] : cpx<T>{T(0), T(0)};
+    }
     for (int i = tid; i < m2; i += PC_NT) tw[i] = a.tw2[i];
 
     if (mode != PZ_BEGIN) {
-        for (int e = tid; e < ne; e += PC_NT) {
-            const int l = e / m2, z = e - l * m2;
-            const bool ok = l < nl;
-            buf[z * P + l] = ok ? a.Ch[fo + e] : cpx<T>{T(0), T(0)};
-            if (nfl == 2) buf[z * P + nlz + l] = ok ? a.Cf[fo + e] : cpx<T>{T(0), T(0)};
+#pragma unroll
+        for (int i = 0; i < PZ_EPT; ++i) {
+            const int e = tid + i * PC_NT;
+            if (e < nt) {
+                const int l = e / m2, z = e - l * m2;
+                buf[z * P + l] = c0[i];
+                if (nfl == 2) buf[z * P + nlz + l] = c1[i];
+            }
         }
         __syncthreads();
         pc_fft<T, true>(buf, m2, nlines, P, tw, tid);
@@ -126,20 +150,25 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
 
     if (mode == PZ_MID) {
         const bool qa = s < a.n1, pb = s < a.n2;  // species of this step (density.h:136-146)
-        for (int e = tid; e < ne; e += PC_NT) {
-            const int l = e / m2, z = e - l * m2;
-            if (l < nl) {
-                const T va = a.e4A[fo + e], vb = a.e4B[fo + e];
+#pragma unroll
+        for (int i = 0; i < PZ_EPT; ++i) {
+            const int e = tid + i * PC_NT;
+            if (e < ne) {
+                const int l = e / m2, z = e - l * m2;
                 cpx<T>& v = buf[z * P + l];
-                v.x = Boltz<T>::mul2(v.x, qa ? va : vb);
-                v.y = Boltz<T>::mul2(v.y, pb ? vb : va);
+                v.x = Boltz<T>::mul2(v.x, qa ? va[i] : vb[i]);
+                v.y = Boltz<T>::mul2(v.y, pb ? vb[i] : va[i]);
             }
         }
         __syncthreads();
         pc_fft<T, false>(buf, m2, nlines, P, tw, tid);
-        for (int e = tid; e < ne; e += PC_NT) {
-            const int l = e / m2, z = e - l * m2;
-            if (l < nl) a.Ch[fo + e] = buf[z * P + l];
+#pragma unroll
+        for (int i = 0; i < PZ_EPT; ++i) {
+            const int e = tid + i * PC_NT;
+            if (e < ne) {
+                const int l = e / m2, z = e - l * m2;
+                a.Ch[fo + e] = buf[z * P + l];
+            }
         }
         return;
     }
@@ -148,35 +177,40 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
     cpx<T>* qo = a.qs + ((size_t)slot * (size_t)(a.N + 1) + (size_t)sout) * (size_t)a.M + (size_t)l0 * m2;
     const bool qa = s < a.n1, pb = s < a.n2;        // this step's species
     const bool na = sout < a.n1, nbb = sout < a.n2;  // next step's species
-    for (int e = tid; e < ne; e += PC_NT) {
-        const int l = e / m2, z = e - l * m2;
-        if (l < nl) {
-            const T va = a.e4A[fo + e], vb = a.e4B[fo + e];
-            cpx<T> q;
-            if (mode == PZ_BEGIN) {
-                q = {T(1), T(1)};
-            } else {
-                const cpx<T> h0 = buf[z * P + l], h1 = buf[z * P + nlz + l];
-                q.x = Boltz<T>::rich(h0.x, h1.x, qa ? va : vb);
-                q.y = Boltz<T>::rich(h0.y, h1.y, pb ? vb : va);
+#pragma unroll
+    for (int i = 0; i < PZ_EPT; ++i) {
+        const int e = tid + i * PC_NT;
+        if (e < nt) {
+            const int l = e / m2, z = e - l * m2;
+            if (e < ne) {
+                cpx<T> q;
+                if (mode == PZ_BEGIN) {
+                    q = {T(1), T(1)};
+                } else {
+                    const cpx<T> h0 = buf[z * P + l], h1 = buf[z * P + nlz + l];
+                    q.x = Boltz<T>::rich(h0.x, h1.x, qa ? va[i] : vb[i]);
+                    q.y = Boltz<T>::rich(h0.y, h1.y, pb ? vb[i] : va[i]);
+                }
+                qo[e] = q;
+                if (mode != PZ_LAST) {
+                    const T ex = na ? va[i] : vb[i], ey = nbb ? vb[i] : va[i];
+                    buf[z * P + l] = {Boltz<T>::mul4(q.x, ex), Boltz<T>::mul4(q.y, ey)};
+                    buf[z * P + nlz + l] = {Boltz<T>::mul2(q.x, ex), Boltz<T>::mul2(q.y, ey)};
+                }
+            } else if (mode == PZ_BEGIN) {
+                buf[z * P + l] = {T(0), T(0)};
+                buf[z * P + nlz + l] = {T(0), T(0)};
             }
-            qo[e] = q;
-            if (mode != PZ_LAST) {
-                const T ex = na ? va : vb, ey = nbb ? vb : va;
-                buf[z * P + l] = {Boltz<T>::mul4(q.x, ex), Boltz<T>::mul4(q.y, ey)};
-                buf[z * P + nlz + l] = {Boltz<T>::mul2(q.x, ex), Boltz<T>::mul2(q.y, ey)};
-            }
-        } else if (mode == PZ_BEGIN) {
-            buf[z * P + l] = {T(0), T(0)};
-            buf[z * P + nlz + l] = {T(0), T(0)};
         }
     }
     if (mode == PZ_LAST) return;
     __syncthreads();
     pc_fft<T, false>(buf, m2, nlines, P, tw, tid);
-    for (int e = tid; e < ne; e += PC_NT) {
-        const int l = e / m2, z = e - l * m2;
-        if (l < nl) {
+#pragma unroll
+    for (int i = 0; i < PZ_EPT; ++i) {
+        const int e = tid + i * PC_NT;
+        if (e < ne) {
+            const int l = e / m2, z = e - l * m2;
             a.Ch[fo + e] = buf[z * P + l];
             a.Cf[fo + e] = buf[z * P + nlz + l];
         }
@@ -204,17 +238,28 @@ __global__ void __launch_bounds__(PC_NT) k_py(PropArgs<T> a, int nh, int tz)
     const int x = blockIdx.x / ntz, z0 = (blockIdx.x - x * ntz) * tz;
     const int nz = m2 - z0 < tz ? m2 - z0 : tz;
     cpx<T>* g = pc_field(a, blockIdx.y, nh) + (size_t)x * m1 * m2 + z0;
-    for (int i = tid; i < m1; i += PC_NT) tw[i] = a.tw1[i];
     const int ne = m1 * tz;
-    for (int e = tid; e < ne; e += PC_NT) {
+    cpx<T> r[PC_EPT];
+#pragma unroll
+    for (int i = 0; i < PC_EPT; ++i) {
+        const int e = tid + i * PC_NT;
         const int y = e / tz, zz = e - y * tz;
-        buf[y * P + zz] = zz < nz ? g[(size_t)y * m2 + zz] : cpx<T>{T(0), T(0)};
+        r[i] = (e < ne && zz < nz) ? g[(size_t)y * m2 + zz] : cpx<T>{T(0), T(0)};
+    }
+    for (int i = tid; i < m1; i += PC_NT) tw[i] = a.tw1[i];
+#pragma unroll
+    for (int i = 0; i < PC_EPT; ++i) {
+        const int e = tid + i * PC_NT;
+        const int y = e / tz, zz = e - y * tz;
+        if (e < ne) buf[y * P + zz] = r[i];
     }
     __syncthreads();
     pc_fft<T, INV>(buf, m1, tz, P, tw, tid);
-    for (int e = tid; e < ne; e += PC_NT) {
+#pragma unroll
+    for (int i = 0; i < PC_EPT; ++i) {
+        const int e = tid + i * PC_NT;
         const int y = e / tz, zz = e - y * tz;
-        if (zz < nz) g[(size_t)y * m2 + zz] = buf[y * P + zz];
+        if (e < ne && zz < nz) g[(size_t)y * m2 + zz] = buf[y * P + zz];
     }
 }
 
@@ -235,24 +280,38 @@ __global__ void __launch_bounds__(PC_NT) k_px(PropArgs<T> a, int nh, int tx)
     const int nz = m2 - z0 < tx ? m2 - z0 : tx;
     const bool full = (int)blockIdx.y >= nh;
     cpx<T>* g = pc_field(a, blockIdx.y, nh) + (size_t)y * m2 + z0;
-    for (int i = tid; i < m0; i += PC_NT) tw[i] = a.tw0[i];
     const int ne = m0 * tx;
-    for (int e = tid; e < ne; e += PC_NT) {
+    cpx<T> r[PC_EPT];
+#pragma unroll
+    for (int i = 0; i < PC_EPT; ++i) {
+        const int e = tid + i * PC_NT;
         const int x = e / tx, zz = e - x * tx;
-        buf[x * P + zz] = zz < nz ? g[x * sx + zz] : cpx<T>{T(0), T(0)};
+        r[i] = (e < ne && zz < nz) ? g[x * sx + zz] : cpx<T>{T(0), T(0)};
+    }
+    for (int i = tid; i < m0; i += PC_NT) tw[i] = a.tw0[i];
+#pragma unroll
+    for (int i = 0; i < PC_EPT; ++i) {
+        const int e = tid + i * PC_NT;
+        const int x = e / tx, zz = e - x * tx;
+        if (e < ne) buf[x * P + zz] = r[i];
     }
     __syncthreads();
     pc_fft<T, false>(buf, m0, tx, P, tw, tid);
     const T gy = a.gy[y];
-    for (int e = tid; e < ne; e += PC_NT) {
+#pragma unroll
+    for (int i = 0; i < PC_EPT; ++i) {
+        const int e = tid + i * PC_NT;
         const int x = e / tx, zz = e - x * tx;
-        if (zz < nz) Kinetic<T>::apply(buf[x * P + zz], a.gx[x], Kinetic<T>::col(gy, a.gz[z0 + zz]), full, a.invM);
+        if (e < ne && zz < nz)
+            Kinetic<T>::apply(buf[x * P + zz], a.gx[x], Kinetic<T>::col(gy, a.gz[z0 + zz]), full, a.invM);
     }
     __syncthreads();
     pc_fft<T, true>(buf, m0, tx, P, tw, tid);
-    for (int e = tid; e < ne; e += PC_NT) {
+#pragma unroll
+    for (int i = 0; i < PC_EPT; ++i) {
+        const int e = tid + i * PC_NT;
         const int x = e / tx, zz = e - x * tx;
-        if (zz < nz) g[x * sx + zz] = buf[x * P + zz];
+        if (e < ne && zz < nz) g[x * sx + zz] = buf[x * P + zz];
     }
 }
 
