@@ -72,6 +72,8 @@ def lib():
         L.orc_free_energy.restype = C.c_double
         L.orc_free_energy.argtypes = [vp, C.POINTER(OrcRun), C.POINTER(dp), C.POINTER(dp), ip, dp,
                                       dp, dp, dp, dp]
+        L.orc_dlnq_dbox.argtypes = [vp, dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                    dp, dp]
         _LIB = L
     return _LIB
 
@@ -147,6 +149,17 @@ class Grid:
         if want_q:
             out["q"] = qs.reshape(N + 1, 2, M)
         return out
+
+    def dlnq_dbox(self, W, N, alpha, phic, f, ens):
+        """Exact d lnQ / dD_i (i = x, y, z) of the discrete props at fixed W: (lnQ, (3,))."""
+        M = self.M
+        w = np.zeros(6 * M)
+        w[: 2 * M] = np.asarray(W, dtype=np.float64).reshape(-1)
+        lnQ = C.c_double()
+        d = np.zeros(3)
+        lib().orc_dlnq_dbox(self._h, _dp(w), int(N), float(alpha), float(phic), float(f), int(ens),
+                            C.byref(lnQ), _dp(d))
+        return lnQ.value, d
 
     def solve_field0(self, Wrow, DEV, chi, f, N, alpha, phic, ens, maxIter, errTol):
         return lib().orc_solve_field0(self._h, _dp(Wrow), _dp(DEV), float(chi), float(f), int(N),

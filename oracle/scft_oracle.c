@@ -712,3 +712,120 @@ double orc_free_energy(orc_grid *g, const orc_run *run, double **W, double **DEV
     for (int ii = 0; ii < strn; ii++) FE += FEs[ii];
     return FE;
 }
+
+/* ------------------------------------------------------------------------------------ */
+/* Box derivative of lnQ at fixed W (the adjoint of density.h:344-399)                  */
+/* ------------------------------------------------------------------------------------ */
+/* The step operator S = (4 E4 K12 E2 K12 E4 - E2 K6 E2)/3 (density.h:129-220) is symmetric,
+ * so the adjoint of q1 after step s is q2[N-1-s] and
+ *   dQC/dD_i = sum_s <q2[N-1-s], dS_s/dD_i q1[s]>,  <f, g> = mean(f g).
+ * With v0 = E4 b, u0 = E4 a, v1 = E2 K12 v0, u1 = E2 K12 u0:
+ *   <a, dS b> = (4 (<u0, dK12 v1> + <u1, dK12 v0>) - <E2 a, dK6 E2 b>) / 3,
+ * and, from k^2 = sum_j (2 pi n_j / D_j)^2 (density.h:79-95),
+ *   dK12/dD_i = K12 * k_i^2 ds / (6 D_i),  dK6/dD_i = K6 * k_i^2 ds / (3 D_i).
+ * Each <u, dK v> is taken in real space: dK applied by an FFT sandwich, then a mean of
+ * products. Q2 = <q2[Nh]> uses q2 itself as its adjoint the same way. */
+static void kcomp(const orc_grid *g, int dir, double *out)
+{
+    const int *m = g->m;
+    const double pi = 4 * atan(1.0);
+    for (int k0 = -(m[0] - 1) / 2; k0 <= m[0] / 2; k0++) {
+        const int K0 = (k0 < 0) ? (k0 + m[0]) : k0;
+        for (int k1 = -(m[1] - 1) / 2; k1 <= m[1] / 2; k1++) {
+            const int K1 = (k1 < 0) ? (k1 + m[1]) : k1;
+            for (int k2 = 0; k2 <= m[2] / 2; k2++) {
+                const int n = dir == 0 ? k0 : (dir == 1 ? k1 : k2);
+                const int k = k2 + (m[2] / 2 + 1) * (K1 + m[1] * K0);
+                out[k] = 4 * pi * pi * (double)n * n / (g->D[dir] * g->D[dir]);
+            }
+        }
+    }
+}
+
+static double mean_prod(const double *a, const double *b, int M)
+{
+    double s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (int r = 0; r < M; r++) s += a[r] * b[r];
+    return s / M;
+}
+
+/* adds <a, dS/dD_i b> for i = 0..2 to acc */
+static void grad_term(orc_grid *g, const double *e4, const double *e2, double *const dk12[3],
+                      double *const dk6[3], const double *a, const double *b, double acc[3])
+{
+    const int M = g->M;
+    double *v0 = malloc(sizeof(double) * M), *u0 = malloc(sizeof(double) * M);
+    double *v1 = malloc(sizeof(double) * M), *u1 = malloc(sizeof(double) * M);
+    double *vf = malloc(sizeof(double) * M), *uf = malloc(sizeof(double) * M);
+    double *t = malloc(sizeof(double) * M);
+    for (int r = 0; r < M; r++) {
+        v0[r] = e4[r] * b[r];
+        u0[r] = e4[r] * a[r];
+        vf[r] = e2[r] * b[r];
+        uf[r] = e2[r] * a[r];
+    }
+    memcpy(v1, v0, sizeof(double) * M);
+    memcpy(u1, u0, sizeof(double) * M);
+    sandwich(g, v1, g->k12);
+    sandwich(g, u1, g->k12);
+    for (int r = 0; r < M; r++) { v1[r] *= e2[r]; u1[r] *= e2[r]; }
+    for (int i = 0; i < 3; i++) {
+        double th = 0, tf;
+        memcpy(t, v1, sizeof(double) * M);
+        sandwich(g, t, dk12[i]);
+        th += mean_prod(u0, t, M);
+        memcpy(t, v0, sizeof(double) * M);
+        sandwich(g, t, dk12[i]);
+        th += mean_prod(u1, t, M);
+        memcpy(t, vf, sizeof(double) * M);
+        sandwich(g, t, dk6[i]);
+        tf = mean_prod(uf, t, M);
+        acc[i] += (4 * th - tf) / 3;
+    }
+    free(v0); free(u0); free(v1); free(u1); free(vf); free(uf); free(t);
+}
+
+void orc_dlnq_dbox(orc_grid *g, double *w, int Ns, double alpha, double phic, double f, int ens,
+                   double *lnQ, double dlnQ[3])
+{
+    const int M = g->M, Mk = g->Mk;
+    const int n1 = (int)round(Ns * f), n2 = Ns - n1;
+    const int Nh = (int)(alpha * Ns); /* density.h:346 */
+    const double ds = 1.0 / Ns;
+    double Q[3], phis[5];
+    orc_props(g, w, lnQ, phis, Ns, alpha, phic, f, ens, Q, NULL);
+    const double Q2 = Q[1], QC = Q[2];
+#define Q1P(s) (g->qq + (size_t)2 * (s) * M)
+#define Q2P(s) (g->qq + ((size_t)2 * (s) + 1) * M)
+    double *dk12[3], *dk6[3], *kc = malloc(sizeof(double) * Mk);
+    for (int i = 0; i < 3; i++) {
+        dk12[i] = malloc(sizeof(double) * Mk);
+        dk6[i] = malloc(sizeof(double) * Mk);
+        kcomp(g, i, kc);
+        for (int k = 0; k < Mk; k++) {
+            dk12[i][k] = g->k12[k] * kc[k] * ds / (6 * g->D[i]);
+            dk6[i][k] = g->k6[k] * kc[k] * ds / (3 * g->D[i]);
+        }
+    }
+    double dQC[3] = {0, 0, 0}, dQ2[3] = {0, 0, 0};
+    for (int s = 0; s < Ns; s++) {
+        const int isA = s < n1; /* species of chain 1 on step s (density.h:136-146) */
+        grad_term(g, isA ? g->e4A : g->e4B, isA ? g->e2A : g->e2B, dk12, dk6, Q2P(Ns - 1 - s),
+                  Q1P(s), dQC);
+    }
+    for (int t = 0; t < Nh; t++) {
+        const int isB = t < n2; /* species of chain 2 on step t */
+        grad_term(g, isB ? g->e4B : g->e4A, isB ? g->e2B : g->e2A, dk12, dk6, Q2P(Nh - 1 - t),
+                  Q2P(t), dQ2);
+    }
+    for (int i = 0; i < 3; i++) {
+        if (ens == 1) dlnQ[i] = phic * dQC[i] / QC + ((1.0 - phic) / alpha) * dQ2[i] / Q2;
+        else dlnQ[i] = dQ2[i] + phic * dQC[i];
+        free(dk12[i]);
+        free(dk6[i]);
+    }
+    free(kc);
+#undef Q1P
+#undef Q2P
+}

@@ -23,6 +23,7 @@
  *   orc_string_iter         straight-line string fixed point, projection invariant
  *                           (parity of the control flow itself: transcription only)
  *   orc_free_energy         disordered-phase closed form
+ *   orc_dlnq_dbox           central differences of orc_props in D; zero for a uniform field
  */
 #ifndef SCFT_ORACLE_H
 #define SCFT_ORACLE_H
@@ -106,6 +107,12 @@ void orc_string_iter(orc_grid *g, const orc_run *run, double **W, double **DEV, 
 double orc_free_energy(orc_grid *g, const orc_run *run, double **W, double **DEV,
                        const int *doiis, double *alf, double *FEs0, double *FEs1, double *FEs,
                        double *phctot);
+
+/* Exact derivative of the discrete lnQ of density::props (density.h:344-399) with respect
+ * to the box lengths D[0..2] at fixed W, from the stored q1, q2 as adjoints. w as in
+ * orc_props (phi out). dlnQ[3] out. Pinned by central differences of orc_props. */
+void orc_dlnq_dbox(orc_grid *g, double *w, int Ns, double alpha, double phic, double f, int ens,
+                   double *lnQ, double dlnQ[3]);
 
 #ifdef __cplusplus
 }
