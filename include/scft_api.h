@@ -121,6 +121,25 @@ scft_status scft_string_iterate(scft_ctx *ctx, int k, double errTol, double *alf
 scft_status scft_free_energy(scft_ctx *ctx, double *FE, double *FEs, double *FEs0,
                              double *FEs1, double *phctot, double *alf);
 
+/* Box derivative of lnQ (density.h:344-399) at fixed W for nrow rows W[nrow][2M]:
+ * lnQ[nrow] (may be NULL) and dlnQ_dD[nrow][3] out. The exact derivative of the discrete
+ * Richardson propagator: the stored q1 and q2 serve as each other's adjoints. Uses
+ * scratch state only. The reference has no counterpart (it computes no gradients); this
+ * is the backward pass for stress-free box relaxation. */
+scft_status scft_dlnq_dbox(scft_ctx *ctx, const double *W, int nrow, double *lnQ,
+                           double *dlnQ_dD);
+
+/* Free-energy derivatives at the current field state, per replica (props on every row):
+ * dFdD[nrep][3] = -d lnQ/dD_i, the box stress, and dFdchi[nrep] = <phiA phiB>, the chiN
+ * derivative of FEs (scft.cu:823-828). Both are the total derivatives of FreeE's F at a
+ * saddle point, where F is stationary in W. Either output may be NULL. No counterpart in
+ * the reference. */
+scft_status scft_box_stress(scft_ctx *ctx, double *dFdD, double *dFdchi);
+
+/* New box lengths D[3] for every later call: rebuilds the kinetic tables (setup_k_sq,
+ * density.h:79-95, which the reference runs once at start-up). */
+scft_status scft_set_box(scft_ctx *ctx, const double D[3]);
+
 /* Diagnostics for tests: after a props call on slot `slot` of the scratch batch,
  * qq[s] = <q1'[s] q2'[N-s]> for s = 0..N (the A4 symmetry invariant, with the
  * mean-field factoring left in; all entries equal QC' in exact arithmetic). */

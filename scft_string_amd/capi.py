@@ -18,7 +18,8 @@ EXPORTED = [
     "scft_ctx_create", "scft_ctx_destroy", "scft_last_error", "scft_sync", "scft_props_batch",
     "scft_props_batch_device", "scft_set_fields", "scft_get_fields", "scft_get_densities",
     "scft_get_tangent", "scft_solve_w_plus", "scft_string_iterate", "scft_free_energy",
-    "scft_debug_qq_invariant", "scft_debug_time_props", "scft_fft_path",
+    "scft_debug_qq_invariant", "scft_debug_time_props", "scft_fft_path", "scft_dlnq_dbox",
+    "scft_box_stress", "scft_set_box",
 ]
 _STRING_RESULT = ("scft_last_error", "scft_fft_path")
 
@@ -75,6 +76,9 @@ def lib():
         L.scft_debug_time_props.argtypes = [vp, C.c_int, C.c_int, dp]
         L.scft_fft_path.argtypes = [vp]
         L.scft_fft_path.restype = C.c_char_p
+        L.scft_dlnq_dbox.argtypes = [vp, dp, C.c_int, dp, dp]
+        L.scft_box_stress.argtypes = [vp, dp, dp]
+        L.scft_set_box.argtypes = [vp, dp]
         for name in EXPORTED:
             if name not in _STRING_RESULT:
                 getattr(L, name).restype = C.c_int
@@ -93,6 +97,7 @@ class Context:
     def __init__(self, m, D, N, f, alpha, ens, phidb, chi, nrep=1, precision="fp64", fix=(0, 0),
                  dostring=1, justFE=0, springsteps=0, device=0, fe_fp64=1):
         self.m = tuple(int(v) for v in m)
+        self.D = tuple(float(v) for v in D)
         self.M = int(np.prod(self.m))
         self.nrep = int(nrep)
         self.N = int(N)
@@ -215,3 +220,25 @@ class Context:
         FEs, FEs0, FEs1, ph, alf = (np.zeros(R) for _ in range(5))
         self._chk(lib().scft_free_energy(self._h, C.byref(FE), _dp(FEs), _dp(FEs0), _dp(FEs1), _dp(ph), _dp(alf)))
         return {"FE": FE.value, "FEs": FEs, "FEs0": FEs0, "FEs1": FEs1, "phctot": ph, "alf": alf}
+
+    # ---------------------------------------------------------------- box derivatives
+    def dlnq_dbox(self, W):
+        """d lnQ / dD_i at fixed W for rows W (n, 2, M): returns lnQ (n,), dlnQ_dD (n, 3)."""
+        W = np.ascontiguousarray(W, dtype=np.float64).reshape(-1, 2 * self.M)
+        n = W.shape[0]
+        lnQ = np.empty(n)
+        d = np.empty((n, 3))
+        self._chk(lib().scft_dlnq_dbox(self._h, _dp(W), n, _dp(lnQ), _dp(d)))
+        return lnQ, d
+
+    def box_stress(self):
+        """dF/dD (nrep, 3) and dF/dchiN (nrep,) at the current field state."""
+        dFdD = np.empty((self.nrep, 3))
+        dFdchi = np.empty(self.nrep)
+        self._chk(lib().scft_box_stress(self._h, _dp(dFdD), _dp(dFdchi)))
+        return dFdD, dFdchi
+
+    def set_box(self, D):
+        D = np.ascontiguousarray(D, dtype=np.float64).reshape(3)
+        self._chk(lib().scft_set_box(self._h, _dp(D)))
+        self.D = tuple(float(v) for v in D)

@@ -356,4 +356,24 @@ void launch_fe_sums(const int* d_slot2rep, int nb, long long M, double chi, cons
     hipLaunchKernelGGL(k_fe_sums, dim3(nblk, nb), dim3(FK_NT), 0, st, d_slot2rep, M, chi, W, phi, part);
 }
 
+// part[slot][nblk] = block sums of phiA phiB (the chi derivative of FEs1, scft.cu:823-828)
+__global__ void __launch_bounds__(FK_NT) k_phiab_sums(const int* __restrict__ slot2rep, long long M,
+                                                      const double* __restrict__ phi, double* __restrict__ part)
+{
+    __shared__ double sh[FK_NT];
+    const int slot = blockIdx.y;
+    const double* ph = phi + (size_t)slot2rep[slot] * 4 * M;
+    double s = 0;
+    GRID_STRIDE(r, M) { s += ph[r] * ph[M + r]; }
+    s = fk_block_sum(s, sh);
+    if (threadIdx.x == 0) part[(size_t)slot * gridDim.x + blockIdx.x] = s;
+}
+
+void launch_phiab_sums(const int* d_slot2rep, int nb, long long M, const double* phi, double* part, int nblk,
+                       hipStream_t st)
+{
+    if (nb <= 0) return;
+    hipLaunchKernelGGL(k_phiab_sums, dim3(nblk, nb), dim3(FK_NT), 0, st, d_slot2rep, M, phi, part);
+}
+
 }  // namespace scft

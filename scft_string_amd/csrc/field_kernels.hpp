@@ -50,4 +50,8 @@ void launch_redistribute(int R, long long M, const double* d_x, const double* d_
 void launch_fe_sums(const int* d_slot2rep, int nb, long long M, double chi, const double* W, const double* phi,
                     double* part, int nblk, hipStream_t st);
 
+// part [slot][nblk] = block sums of phiA phiB
+void launch_phiab_sums(const int* d_slot2rep, int nb, long long M, const double* phi, double* part, int nblk,
+                       hipStream_t st);
+
 }  // namespace scft

@@ -3,6 +3,7 @@
 #include "prop_kernels.hpp"
 #include "prop_kernels2.hpp"
 #include "prop_pencil.hpp"
+#include "grad_kernels.hpp"
 
 #include <cmath>
 #include <cstdio>
@@ -214,7 +215,8 @@ static void pencil_step(const PropArgs<T>& a, int s, hipStream_t st)
 template <typename T>
 PropEngine<T>::~PropEngine()
 {
-    void* ptrs[] = {Ch_, Cf_, e4A_, e4B_, qs_, tw_[0], tw_[1], tw_[2], g_[0], g_[1], g_[2], part_, cAB_, scal_};
+    void* ptrs[] = {Ch_,   Cf_,   e4A_,  e4B_,  qs_, tw_[0], tw_[1], tw_[2], g_[0],
+                    g_[1], g_[2], part_, cAB_, scal_, K0_, K1_,    Kf_,    gpart_};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : ev_)
@@ -356,6 +358,28 @@ hipError_t PropEngine<T>::run(const double* dW, long long wstride, double* dphi,
                               const int* d_slot2rep, int nb, const ChainConst& cc, double* d_phis, hipStream_t st)
 {
     if (nb <= 0) return hipSuccess;
+    const PropArgs<T> a = args(nb, cc);
+
+    const dim3 rg(nblk_, nb);
+    hipLaunchKernelGGL(k_wsum, rg, dim3(RED_NT), 0, st, dW, wstride, d_slot2rep, M_, part_);
+    hipLaunchKernelGGL(k_wmean, dim3(nb), dim3(RED_NT), 0, st, part_, nblk_, M_, cAB_);
+    hipLaunchKernelGGL((k_factors<T>), rg, dim3(RED_NT), 0, st, dW, wstride, d_slot2rep, M_, cc.ds, cAB_, e4A_,
+                       e4B_);
+    if (timing_) (void)hipEventRecord(ev_[0], st);
+    hipError_t e = contour(a, nb, cc, st);
+    if (e != hipSuccess) return e;
+    if (timing_) (void)hipEventRecord(ev_[1], st);
+    hipLaunchKernelGGL((k_qsum<T>), rg, dim3(RED_NT), 0, st, qs_, M_, cc.N, cc.Nh, part_);
+    hipLaunchKernelGGL(k_scalars, dim3(nb), dim3(RED_NT), 0, st, part_, nblk_, M_, cc, cAB_, scal_);
+    hipLaunchKernelGGL((k_density<T>), rg, dim3(RED_NT), 0, st, qs_, M_, cc, scal_, dphi, phistride, d_slot2rep,
+                       part_);
+    hipLaunchKernelGGL(k_phis, dim3(nb), dim3(RED_NT), 0, st, part_, nblk_, M_, d_phis);
+    return hipGetLastError();
+}
+
+template <typename T>
+PropArgs<T> PropEngine<T>::args(int nb, const ChainConst& cc) const
+{
     PropArgs<T> a;
     a.M = M_;
     a.m0 = m_[0];
@@ -378,22 +402,78 @@ hipError_t PropEngine<T>::run(const double* dW, long long wstride, double* dphi,
     a.gy = g_[1];
     a.gz = g_[2];
     a.invM = (T)(1.0 / (double)M_);
+    return a;
+}
 
-    const dim3 rg(nblk_, nb);
-    hipLaunchKernelGGL(k_wsum, rg, dim3(RED_NT), 0, st, dW, wstride, d_slot2rep, M_, part_);
-    hipLaunchKernelGGL(k_wmean, dim3(nb), dim3(RED_NT), 0, st, part_, nblk_, M_, cAB_);
-    hipLaunchKernelGGL((k_factors<T>), rg, dim3(RED_NT), 0, st, dW, wstride, d_slot2rep, M_, cc.ds, cAB_, e4A_,
-                       e4B_);
-    if (timing_) (void)hipEventRecord(ev_[0], st);
-    hipError_t e = contour(a, nb, cc, st);
-    if (e != hipSuccess) return e;
-    if (timing_) (void)hipEventRecord(ev_[1], st);
-    hipLaunchKernelGGL((k_qsum<T>), rg, dim3(RED_NT), 0, st, qs_, M_, cc.N, cc.Nh, part_);
-    hipLaunchKernelGGL(k_scalars, dim3(nb), dim3(RED_NT), 0, st, part_, nblk_, M_, cc, cAB_, scal_);
-    hipLaunchKernelGGL((k_density<T>), rg, dim3(RED_NT), 0, st, qs_, M_, cc, scal_, dphi, phistride, d_slot2rep,
-                       part_);
-    hipLaunchKernelGGL(k_phis, dim3(nb), dim3(RED_NT), 0, st, part_, nblk_, M_, d_phis);
-    return hipGetLastError();
+// Box derivative of the factored QC', Q2' (adjoint of props; the pipeline and its maths
+// are described in grad_kernels.hpp). Runs on the line kernels whatever path run() took:
+// the q store and the Boltzmann factors have the same layout on every path.
+template <typename T>
+hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out, hipStream_t st)
+{
+    if (nb <= 0) return hipSuccess;
+    if (!pencil_supported(m_[0], m_[1], m_[2])) return hipErrorNotSupported;
+    hipError_t e;
+    if (!gpart_) {
+        const size_t fld = sizeof(cpx<T>) * (size_t)M_ * nbmax_;
+        const size_t pb = sizeof(double) * (size_t)nbmax_ * 2 * nblk_ * 3;
+        if ((e = hipMalloc((void**)&K0_, fld)) != hipSuccess || (e = hipMalloc((void**)&K1_, fld)) != hipSuccess ||
+            (e = hipMalloc((void**)&Kf_, fld)) != hipSuccess || (e = hipMalloc((void**)&gpart_, pb)) != hipSuccess)
+            return e;
+        bytes_ += 3 * fld + pb;
+        const int lds = (int)pencil_lds_bytes<T>(m_[0], m_[1], m_[2]);
+        const void* fns[] = {(const void*)k_gz<T>, (const void*)k_gx<T>, (const void*)k_py<T, false>,
+                             (const void*)k_py<T, true>};
+        for (const void* f : fns)
+            if ((e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds)) != hipSuccess) return e;
+    }
+    const size_t np = (size_t)nb * nblk_ * 3;  // partials per chain
+    if ((e = hipMemsetAsync(gpart_, 0, sizeof(double) * 2 * np, st)) != hipSuccess) return e;
+
+    const PropArgs<T> a = args(nb, cc);
+    const PencilTiles p = pencil_tiles(m_[0], m_[1], m_[2]);
+    const size_t lds = pencil_lds_bytes<T>(m_[0], m_[1], m_[2]);
+    const double twopi = 8 * atan(1.0);
+    const GradConst gc{cc.ds, {twopi / D_[0], twopi / D_[1], twopi / D_[2]}, {D_[0], D_[1], D_[2]}};
+    const long long lines = (long long)m_[0] * m_[1];
+    const dim3 gzg((unsigned)((lines + p.nlz - 1) / p.nlz), nb);
+    const int ntx = (m_[2] + p.tx - 1) / p.tx;
+    for (int chain = 0; chain < 2; ++chain) {
+        const int ns = chain == 0 ? cc.N : cc.Nh;
+        double* pc = gpart_ + (size_t)chain * np;
+        for (int s = 0; s < ns; ++s) {
+            // chain 0: b = q1[s], a = q2[N-1-s], species of chain 1 on step s (density.h:136-146)
+            // chain 1: b = q2[t], a = q2[Nh-1-t], species of chain 2 on step t
+            const int sa = chain == 0 ? cc.N - 1 - s : cc.Nh - 1 - s;
+            const int pbk = chain == 0 ? 0 : 1;
+            const int spec = chain == 0 ? (s < cc.n1 ? 0 : 1) : (s < cc.n2 ? 1 : 0);
+            hipLaunchKernelGGL((k_gz<T>), gzg, dim3(PC_NT), lds, st, a, (int)GZ_INIT, s, pbk, sa, 1, spec, p.nlz);
+            pencil_py<T, false>(a, nb, 2 * nb, p, lds, st);
+            hipLaunchKernelGGL((k_gx<T>), dim3(m_[1] * ntx, 2 * nb), dim3(PC_NT), lds, st, a, (int)GX_SPLIT, nb,
+                               p.tx, K0_, Kf_);
+            pencil_py<T, true>(a, nb, nb, p, lds, st);
+            hipLaunchKernelGGL((k_gz<T>), gzg, dim3(PC_NT), lds, st, a, (int)GZ_MID, 0, 0, 0, 0, spec, p.nlz);
+            pencil_py<T, false>(a, nb, nb, p, lds, st);
+            hipLaunchKernelGGL((k_gx<T>), dim3(m_[1] * ntx, nb), dim3(PC_NT), lds, st, a, (int)GX_FWD, nb, p.tx,
+                               K1_, (cpx<T>*)nullptr);
+            hipLaunchKernelGGL((k_gacc<T>), dim3(nblk_, nb), dim3(RED_NT), 0, st, K0_, K1_, Kf_, M_, m_[0], m_[1],
+                               m_[2], gc, pc);
+        }
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    std::vector<double> h(2 * np);
+    if ((e = hipMemcpyAsync(h.data(), gpart_, sizeof(double) * 2 * np, hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    const double scale = 1.0 / ((double)M_ * (double)M_);
+    for (int slot = 0; slot < nb; ++slot)
+        for (int chain = 0; chain < 2; ++chain)
+            for (int i = 0; i < 3; ++i) {
+                double v = 0;
+                for (int b = 0; b < nblk_; ++b) v += h[(size_t)chain * np + ((size_t)slot * nblk_ + b) * 3 + i];
+                out[(size_t)slot * 6 + chain * 3 + i] = v * scale;
+            }
+    return hipSuccess;
 }
 
 // The contour loop (begin + N steps) for nb slots. Replayed from a hipGraph captured once

@@ -53,6 +53,10 @@ public:
     const double* scal() const { return scal_; }
     // qq[s] = <q1'[s] q2'[N-s]> for s = 0..N of one slot (host array of N+1)
     hipError_t qq_invariant(int slot, double* qq, hipStream_t st);
+    // Box derivatives of the factored partition functions of the nb slots of the last run()
+    // (same cc): out[slot][0..2] = dQC'/dD_i, out[slot][3..5] = dQ2'/dD_i (host array).
+    // hipErrorNotSupported when an axis does not factor into 2, 3, 5 or exceeds 1024.
+    hipError_t box_gradient(int nb, const ChainConst& cc, double* out, hipStream_t st);
     const cpx<T>* qstore() const { return qs_; }
     long long M() const { return M_; }
     int N() const { return N_; }
@@ -65,6 +69,7 @@ public:
     double loop_ms() const;
 
 private:
+    PropArgs<T> args(int nb, const ChainConst& cc) const;
     hipError_t contour(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st);
     hipError_t contour_launch(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st);
     struct GraphEntry {
@@ -91,6 +96,11 @@ private:
     cpx<T>* tw_[3] = {nullptr, nullptr, nullptr};
     T* g_[3] = {nullptr, nullptr, nullptr};
     double* part_ = nullptr;
+    // box-gradient spectra and partial sums (allocated on first use)
+    cpx<T>* K0_ = nullptr;
+    cpx<T>* K1_ = nullptr;
+    cpx<T>* Kf_ = nullptr;
+    double* gpart_ = nullptr;
     double* cAB_ = nullptr;
     double* scal_ = nullptr;
     size_t bytes_ = 0;

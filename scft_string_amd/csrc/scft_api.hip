@@ -567,6 +567,94 @@ scft_status scft_free_energy(scft_ctx* c, double* FE, double* FEs, double* FEs0,
     return SCFT_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// d lnQ / dD_i of the nb slots of the last run_props on engine pe: from the box
+// derivatives of the factored QC', Q2' and the ensemble's lnQ (density.h:388-399).
+template <typename T>
+scft_status dlnq_from_engine(scft_ctx* c, PropEngine<T>* pe, int nb, double* dl)
+{
+    std::vector<double> d(6 * (size_t)nb), sc(8 * (size_t)nb);
+    hipError_t e = pe->box_gradient(nb, c->cc, d.data(), c->st);
+    if (e == hipErrorNotSupported)
+        return fail(c, SCFT_ERR_UNSUPPORTED, "box gradient: every axis must factor into 2, 3, 5 and be at most 1024");
+    if (e != hipSuccess) return fail(c, SCFT_ERR_HIP, std::string("box gradient: ") + hipGetErrorString(e));
+    HIPCHK(c, hipMemcpyAsync(sc.data(), pe->scal(), sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const double phic = c->cc.phic, alpha = c->cc.alpha;
+    for (int j = 0; j < nb; ++j) {
+        const double QC = sc[8 * j + 3], Q2 = sc[8 * j + 4], QCp = sc[8 * j + 6], Q2p = sc[8 * j + 7];
+        for (int i = 0; i < 3; ++i) {
+            const double rc = d[6 * j + i] / QCp, r2 = d[6 * j + 3 + i] / Q2p;  // d ln QC/dD_i, d ln Q2/dD_i
+            dl[3 * j + i] = c->cc.ens == 1 ? phic * rc + ((1.0 - phic) / alpha) * r2 : Q2 * r2 + phic * QC * rc;
+        }
+    }
+    return SCFT_OK;
+}
+
+scft_status dlnq_slots(scft_ctx* c, int nb, double* dl)
+{
+    return c->pe32 ? dlnq_from_engine(c, c->pe32.get(), nb, dl) : dlnq_from_engine(c, c->pe64.get(), nb, dl);
+}
+
+}  // namespace
+
+extern "C" {
+
+scft_status scft_dlnq_dbox(scft_ctx* c, const double* W, int nrow, double* lnQ, double* dlnQ_dD)
+{
+    if (!c || !W || !dlnQ_dD || nrow <= 0 || nrow > c->R)
+        return fail(c, SCFT_ERR_INVALID, "scft_dlnq_dbox: bad arguments");
+    const long long M = c->M;
+    HIPCHK(c, hipMemcpyAsync(c->dWs, W, sizeof(double) * 2 * M * nrow, hipMemcpyHostToDevice, c->st));
+    std::vector<int> slots(nrow);
+    for (int i = 0; i < nrow; ++i) slots[i] = i;
+    std::vector<double> lq(nrow);
+    SCFTCHK(run_props(c, c->dWs, 2 * M, c->dphis_rows, 4 * M, slots, lq.data(), nullptr));
+    SCFTCHK(dlnq_slots(c, nrow, dlnQ_dD));
+    if (lnQ) std::memcpy(lnQ, lq.data(), sizeof(double) * nrow);
+    return SCFT_OK;
+}
+
+scft_status scft_box_stress(scft_ctx* c, double* dFdD, double* dFdchi)
+{
+    if (!c) return SCFT_ERR_INVALID;
+    const int R = c->R;
+    const long long M = c->M;
+    std::vector<int> slots(R);
+    for (int i = 0; i < R; ++i) slots[i] = i;
+    SCFTCHK(run_props(c, c->dW, 2 * M, c->dphi, 4 * M, slots, nullptr, nullptr));
+    if (dFdchi) {
+        scft::launch_phiab_sums(c->dslot, R, M, c->dphi, c->dpart, c->nblk, c->st);
+        HIPCHK(c, hipGetLastError());
+        std::vector<double> s;
+        SCFTCHK(finalize_to_host(c, R, 1, s));
+        for (int ii = 0; ii < R; ++ii) dFdchi[ii] = s[ii] / (double)M;
+    }
+    if (dFdD) {
+        std::vector<double> dl(3 * (size_t)R);
+        SCFTCHK(dlnq_slots(c, R, dl.data()));
+        for (int i = 0; i < 3 * R; ++i) dFdD[i] = -dl[i];
+    }
+    return SCFT_OK;
+}
+
+scft_status scft_set_box(scft_ctx* c, const double D[3])
+{
+    if (!c || !D || !(D[0] > 0 && D[1] > 0 && D[2] > 0))
+        return fail(c, SCFT_ERR_INVALID, "scft_set_box: box lengths must be positive");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    hipError_t e = hipSuccess;
+    if (c->pe32) e = c->pe32->set_box(D);
+    if (e == hipSuccess && c->pe64) e = c->pe64->set_box(D);
+    if (e == hipSuccess && c->pe64_fe) e = c->pe64_fe->set_box(D);
+    if (e != hipSuccess) return fail(c, SCFT_ERR_HIP, std::string("scft_set_box: ") + hipGetErrorString(e));
+    for (int i = 0; i < 3; ++i) c->p.D[i] = D[i];
+    return SCFT_OK;
+}
+
 scft_status scft_debug_qq_invariant(scft_ctx* c, int slot, double* qq)
 {
     if (!c || !qq || slot < 0 || slot >= c->R) return SCFT_ERR_INVALID;

@@ -1,0 +1,97 @@
+"""Box derivative d lnQ / dD (the adjoint of props) on the GPU, through the C-ABI.
+
+FP64 against the oracle's orc_dlnq_dbox (itself pinned by central differences of
+orc_props): 1e-10 relative to the largest component. The GPU result is also checked
+against 4-point central differences of the GPU's own lnQ, with the box moved by
+scft_set_box. FP32 reports its error.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import orc
+from scft_string_amd import Context
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _fields(m, D, seed):
+    rng = np.random.default_rng(seed)
+    M = int(np.prod(m))
+    x = (np.arange(m[0]) * D[0] / m[0])[:, None, None] * np.ones(m)
+    y = (np.arange(m[1]) * D[1] / m[1])[None, :, None] * np.ones(m)
+    z = (np.arange(m[2]) * D[2] / m[2])[None, None, :] * np.ones(m)
+    Wm = (2.0 * np.cos(2 * np.pi * x / D[0]) + np.cos(2 * np.pi * y / D[1])
+          + 0.5 * np.sin(2 * np.pi * z / D[2])).reshape(-1)
+    Wm += 0.3 * rng.standard_normal(M)
+    Wp = 0.2 * rng.standard_normal(M)
+    return np.stack([Wm, Wp])
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+CASES = [
+    # m, D, N, f, alpha, ens, phidb
+    ((16, 16, 16), (4.0, 3.6, 4.4), 40, 0.45, 0.1, 1, 0.9),      # plane kernels for props
+    ((12, 8, 16), (3.1, 2.2, 3.7), 21, 0.4, 0.1, 2, 0.3),
+    ((20, 15, 15), (2.0, 1.5, 1.6), 24, 0.55, 0.2, 1, 0.6),      # line kernels, odd n1
+    ((32, 1, 1), (4.0, 1.0, 1.0), 30, 0.5, 0.1, 1, 0.8),        # 1-D box
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "x".join(map(str, c[0])) + f"_ens{c[5]}")
+def test_dlnq_dbox_fp64_matches_oracle(case):
+    m, D, N, f, alpha, ens, phidb = case
+    _say("dlnq_dbox fp64", case)
+    W = np.stack([_fields(m, D, seed=20 + i) for i in range(3)])
+    with Context(m, D, N, f, alpha, ens, phidb, chi=15.0, nrep=3, precision="fp64") as ctx:
+        lnQ, d = ctx.dlnq_dbox(W)
+    g = orc.Grid(m, D)
+    for i in range(3):
+        rl, rd = g.dlnq_dbox(W[i], N, alpha, phidb, f, ens)
+        assert abs(lnQ[i] - rl) <= 1e-10 * max(1.0, abs(rl))
+        assert _rel(d[i], rd) <= 1e-10, (i, d[i], rd)
+
+
+def test_dlnq_dbox_matches_gpu_central_differences():
+    m, D, N, f, alpha, ens, phidb = (12, 8, 16), (3.1, 2.2, 3.7), 21, 0.4, 0.1, 1, 0.7
+    W = _fields(m, D, seed=4)[None]
+    with Context(m, D, N, f, alpha, ens, phidb, chi=15.0, precision="fp64") as ctx:
+        _, d = ctx.dlnq_dbox(W)
+        fd = np.zeros(3)
+        for i in range(3):
+            h = 1e-3 * D[i]
+            v = []
+            for k in (2, 1, -1, -2):
+                Dk = list(D)
+                Dk[i] += k * h
+                ctx.set_box(Dk)
+                v.append(ctx.props_batch(W)[1][0])
+            fd[i] = (-v[0] + 8 * v[1] - 8 * v[2] + v[3]) / (12 * h)
+        ctx.set_box(D)
+        _, d2 = ctx.dlnq_dbox(W)
+    _say(f"gpu adjoint {d[0]} fd {fd}")
+    assert _rel(d[0], fd) <= 1e-8
+    assert np.array_equal(d, d2)
+
+
+@pytest.mark.parametrize("m", [(16, 16, 16), (20, 15, 15), (64, 64, 64)])
+def test_dlnq_dbox_fp32_error(m):
+    D = tuple(0.25 * v for v in m)
+    N, f, alpha, ens, phidb = 32, 0.5, 0.1, 1, 0.8
+    W = _fields(m, D, seed=6)[None]
+    with Context(m, D, N, f, alpha, ens, phidb, chi=15.0, precision="fp64") as c64:
+        _, d64 = c64.dlnq_dbox(W)
+    with Context(m, D, N, f, alpha, ens, phidb, chi=15.0, precision="fp32") as c32:
+        _, d32 = c32.dlnq_dbox(W)
+    err = _rel(d32, d64)
+    _say(f"dlnq_dbox fp32 {m}: rel error {err:.2e}  ({d32[0]} vs {d64[0]})")
+    assert err <= 1e-4
