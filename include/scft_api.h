@@ -29,7 +29,8 @@ typedef enum scft_status {
     SCFT_ERR_UNSUPPORTED = 2, /* grid size / precision combination without a kernel */
     SCFT_ERR_HIP = 3,         /* HIP runtime error */
     SCFT_ERR_NOMEM = 4,       /* device allocation failed */
-    SCFT_ERR_NAN = 5          /* a field error norm became NaN (scft.cu:539, 770 assert) */
+    SCFT_ERR_NAN = 5,         /* a field error norm became NaN (scft.cu:539, 770 assert) */
+    SCFT_ERR_NOCONV = 6       /* an iteration limit was reached before its tolerance */
 } scft_status;
 
 typedef enum scft_precision {
@@ -139,6 +140,18 @@ scft_status scft_box_stress(scft_ctx *ctx, double *dFdD, double *dFdchi);
 /* New box lengths D[3] for every later call: rebuilds the kinetic tables (setup_k_sq,
  * density.h:79-95, which the reference runs once at start-up). */
 scft_status scft_set_box(scft_ctx *ctx, const double D[3]);
+
+/* Stress-free box relaxation of a single-replica context with dostring = 0. Starting
+ * from box D (in/out), alternately (1) iterates the fields with the solve_field body
+ * (scft_string_iterate, solve_w_plus tolerance 1e-3 as in scft.cu:1123) until errs <= errTol
+ * or maxFieldIter iterations, and (2) moves each box length selected in `axes` (bit i for
+ * D_i) by a secant step on dF/dD_i (first step 1% of D_i, every step at most 10%), until
+ * |dF/dD_i| <= stressTol on every selected axis. dFdD[3] (may be NULL) gets the last
+ * stress, *boxes (may be NULL) the number of boxes visited. Returns SCFT_ERR_NOCONV when
+ * maxBoxes boxes pass without reaching stressTol. No counterpart in the reference. */
+scft_status scft_relax_box(scft_ctx *ctx, int axes, double stressTol, double errTol,
+                           int maxFieldIter, int maxBoxes, double D[3], double *dFdD,
+                           int *boxes);
 
 /* Diagnostics for tests: after a props call on slot `slot` of the scratch batch,
  * qq[s] = <q1'[s] q2'[N-s]> for s = 0..N (the A4 symmetry invariant, with the

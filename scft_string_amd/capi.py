@@ -11,7 +11,8 @@ import numpy as np
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libscft_hip.so")
 
-SCFT_OK, SCFT_ERR_INVALID, SCFT_ERR_UNSUPPORTED, SCFT_ERR_HIP, SCFT_ERR_NOMEM, SCFT_ERR_NAN = range(6)
+(SCFT_OK, SCFT_ERR_INVALID, SCFT_ERR_UNSUPPORTED, SCFT_ERR_HIP, SCFT_ERR_NOMEM, SCFT_ERR_NAN,
+ SCFT_ERR_NOCONV) = range(7)
 SCFT_FP64, SCFT_FP32 = 0, 1
 
 EXPORTED = [
@@ -19,7 +20,7 @@ EXPORTED = [
     "scft_props_batch_device", "scft_set_fields", "scft_get_fields", "scft_get_densities",
     "scft_get_tangent", "scft_solve_w_plus", "scft_string_iterate", "scft_free_energy",
     "scft_debug_qq_invariant", "scft_debug_time_props", "scft_fft_path", "scft_dlnq_dbox",
-    "scft_box_stress", "scft_set_box",
+    "scft_box_stress", "scft_set_box", "scft_relax_box",
 ]
 _STRING_RESULT = ("scft_last_error", "scft_fft_path")
 
@@ -79,6 +80,7 @@ def lib():
         L.scft_dlnq_dbox.argtypes = [vp, dp, C.c_int, dp, dp]
         L.scft_box_stress.argtypes = [vp, dp, dp]
         L.scft_set_box.argtypes = [vp, dp]
+        L.scft_relax_box.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp, ip]
         for name in EXPORTED:
             if name not in _STRING_RESULT:
                 getattr(L, name).restype = C.c_int
@@ -242,3 +244,14 @@ class Context:
         D = np.ascontiguousarray(D, dtype=np.float64).reshape(3)
         self._chk(lib().scft_set_box(self._h, _dp(D)))
         self.D = tuple(float(v) for v in D)
+
+    def relax_box(self, axes=(1, 0, 0), stress_tol=1e-6, err_tol=1e-9, max_field_iter=5000, max_boxes=30):
+        """Stress-free box relaxation (single replica, dostring = 0). Returns (D, dF/dD, boxes)."""
+        mask = sum(1 << i for i in range(3) if axes[i])
+        D = np.ascontiguousarray(self.D, dtype=np.float64)
+        g = np.zeros(3)
+        nbox = C.c_int()
+        self._chk(lib().scft_relax_box(self._h, mask, float(stress_tol), float(err_tol), int(max_field_iter),
+                                       int(max_boxes), _dp(D), _dp(g), C.byref(nbox)))
+        self.D = tuple(float(v) for v in D)
+        return self.D, g, nbox.value

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -653,6 +654,52 @@ scft_status scft_set_box(scft_ctx* c, const double D[3])
     if (e != hipSuccess) return fail(c, SCFT_ERR_HIP, std::string("scft_set_box: ") + hipGetErrorString(e));
     for (int i = 0; i < 3; ++i) c->p.D[i] = D[i];
     return SCFT_OK;
+}
+
+scft_status scft_relax_box(scft_ctx* c, int axes, double stressTol, double errTol, int maxFieldIter, int maxBoxes,
+                           double* D, double* dFdD, int* boxes)
+{
+    if (!c || !D || c->R != 1 || c->p.dostring != 0 || (axes & 7) == 0 || maxFieldIter <= 0 || maxBoxes <= 0)
+        return fail(c, SCFT_ERR_INVALID,
+                    "scft_relax_box: needs one replica, dostring = 0, an axis in `axes` and positive limits");
+    SCFTCHK(scft_set_box(c, D));
+    double Dprev[3] = {0, 0, 0}, gprev[3] = {0, 0, 0}, g[3] = {0, 0, 0};
+    bool have_prev = false;
+    double errs = 0, alf = 0;
+    int steps = 0, k = 0;
+    scft_iter_report rep{};
+    rep.errs = &errs;
+    rep.steps = &steps;
+    for (int b = 0; b < maxBoxes; ++b) {
+        for (int it = 0; it < maxFieldIter; ++it) {
+            SCFTCHK(scft_string_iterate(c, k++, 1e-3, &alf, &rep));
+            if (errs <= errTol) break;
+        }
+        SCFTCHK(scft_box_stress(c, g, nullptr));
+        if (dFdD) std::memcpy(dFdD, g, sizeof g);
+        if (boxes) *boxes = b + 1;
+        bool done = true;
+        for (int i = 0; i < 3; ++i)
+            if ((axes >> i & 1) && std::fabs(g[i]) > stressTol) done = false;
+        if (done) return SCFT_OK;
+        double Dn[3] = {D[0], D[1], D[2]};
+        for (int i = 0; i < 3; ++i) {
+            if (!(axes >> i & 1)) continue;
+            double step = (g[i] > 0 ? -0.01 : 0.01) * D[i];
+            if (have_prev && g[i] != gprev[i]) step = -g[i] * (D[i] - Dprev[i]) / (g[i] - gprev[i]);
+            const double cap = 0.1 * D[i];
+            step = std::max(-cap, std::min(cap, step));
+            Dn[i] = D[i] + step;
+        }
+        for (int i = 0; i < 3; ++i) {
+            Dprev[i] = D[i];
+            gprev[i] = g[i];
+            D[i] = Dn[i];
+        }
+        have_prev = true;
+        SCFTCHK(scft_set_box(c, D));
+    }
+    return fail(c, SCFT_ERR_NOCONV, "scft_relax_box: stress tolerance not reached within maxBoxes boxes");
 }
 
 scft_status scft_debug_qq_invariant(scft_ctx* c, int slot, double* qq)

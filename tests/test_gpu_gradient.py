@@ -95,3 +95,52 @@ def test_dlnq_dbox_fp32_error(m):
     err = _rel(d32, d64)
     _say(f"dlnq_dbox fp32 {m}: rel error {err:.2e}  ({d32[0]} vs {d64[0]})")
     assert err <= 1e-4
+
+
+# ---------------------------------------------------------------- saddle point, relaxation
+LAM = dict(m=(32, 1, 1), N=40, f=0.5, alpha=0.1, ens=1, phidb=0.8, chi=15.0)
+
+
+def _lamella(D0):
+    x = np.arange(32) * D0 / 32
+    return np.stack([2.0 * np.cos(2 * np.pi * x / D0), np.zeros(32)])
+
+
+def test_box_stress_at_saddle_matches_oracle():
+    """scft_box_stress at oracle-converged fields: -d lnQ/dD and <phiA phiB> at 1e-10."""
+    c = LAM
+    D = (4.0, 1.0, 1.0)
+    g = orc.Grid(c["m"], D)
+    W = _lamella(D[0])
+    for _ in range(20000):
+        r = g.props(W, c["N"], c["alpha"], c["phidb"], c["f"], c["ens"])
+        dm = c["chi"] * (r["phi"][1] - r["phi"][0]) - 2 * W[0]
+        dp = c["chi"] * (r["phi"][0] + r["phi"][1] - 1)
+        if max(np.abs(dm).max(), np.abs(dp).max()) < 1e-9:
+            break
+        W[0] += 0.1 * dm
+        W[1] += 0.05 * dp
+    _, dl = g.dlnq_dbox(W, c["N"], c["alpha"], c["phidb"], c["f"], c["ens"])
+    with Context(c["m"], D, c["N"], c["f"], c["alpha"], c["ens"], c["phidb"], c["chi"], dostring=0) as ctx:
+        ctx.set_fields(W[None])
+        dFdD, dFdchi = ctx.box_stress()
+    _say(f"saddle stress gpu {dFdD[0]} oracle {-dl}")
+    assert _rel(dFdD[0], -dl) <= 1e-10
+    ref = np.mean(r["phi"][0] * r["phi"][1])
+    assert abs(dFdchi[0] - ref) <= 1e-10 * ref
+
+
+def test_relax_box_reaches_zero_stress():
+    """Stress-free relaxation of a lamella: |dF/dD| < 1e-6 with the reference's own field
+    iteration. The oracle's stress at the relaxed fields and box confirms the GPU value."""
+    c = LAM
+    with Context(c["m"], (4.0, 1.0, 1.0), c["N"], c["f"], c["alpha"], c["ens"], c["phidb"], c["chi"],
+                 dostring=0) as ctx:
+        ctx.set_fields(_lamella(4.0)[None])
+        D, g, nbox = ctx.relax_box(axes=(1, 0, 0), stress_tol=1e-6, err_tol=1e-9)
+        W = ctx.get_fields()[0]
+    _say(f"relaxed box D = {D[0]!r} after {nbox} boxes, dF/dD = {g[0]:.3e}")
+    assert abs(g[0]) < 1e-6
+    _, dl = orc.Grid(c["m"], D).dlnq_dbox(W.reshape(2, -1), c["N"], c["alpha"], c["phidb"], c["f"], c["ens"])
+    assert abs(-dl[0] - g[0]) <= 1e-10 * max(abs(dl[0]), 1e-3)
+    assert 4.3 < D[0] < 4.7

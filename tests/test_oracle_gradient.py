@@ -5,6 +5,8 @@ applied in real space. It is checked against 4-point central differences of orc_
 itself (error O(h^4), h = 1e-3 D). A uniform field gives exactly zero, because only the
 k = 0 mode is populated and its kinetic factor does not depend on D.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -62,3 +64,70 @@ def test_dlnq_dbox_uniform_field_is_zero():
     W = np.stack([np.full(M, 0.7), np.full(M, -0.2)])
     _, d = orc.Grid(m, D).dlnq_dbox(W, 20, 0.1, 0.8, 0.5, 1)
     assert np.max(np.abs(d)) <= 1e-14
+
+
+# ---------------------------------------------------------------- at a saddle point
+def _converge(g, W, chi, N, f, alpha, phic, ens, tol=1e-10):
+    """Plain mixing to DEV = 0 (any solver reaches the same saddle point)."""
+    for _ in range(100000):
+        r = g.props(W, N, alpha, phic, f, ens)
+        pA, pB = r["phi"][0], r["phi"][1]
+        dm = chi * (pB - pA) - 2 * W[0]
+        dp = chi * (pA + pB - 1)
+        if max(np.abs(dm).max(), np.abs(dp).max()) < tol:
+            return W, r
+        W[0] += 0.1 * dm
+        W[1] += 0.05 * dp
+    raise RuntimeError("no convergence")
+
+
+def _free_energy(W, r, chi):
+    """FreeE's FEs = -lnQ + mean(chi phiA phiB - wA phiA - wB phiB) (scft.cu:823-830)."""
+    pA, pB = r["phi"][0], r["phi"][1]
+    wA, wB = W[1] + W[0], W[1] - W[0]
+    return -r["lnQ"] + np.mean(chi * pA * pB - wA * pA - wB * pB)
+
+
+def _envelope(N, with_chi):
+    m, D0, f, alpha, phic, ens, chi = (16, 1, 1), 4.0, 0.5, 0.1, 0.8, 1, 15.0
+    x = np.arange(m[0]) * D0 / m[0]
+    W = np.stack([2.0 * np.cos(2 * np.pi * x / D0), np.zeros(m[0])])
+    W0, r0 = _converge(orc.Grid(m, (D0, 1.0, 1.0)), W, chi, N, f, alpha, phic, ens)
+    _, dl = orc.Grid(m, (D0, 1.0, 1.0)).dlnq_dbox(W0, N, alpha, phic, f, ens)
+
+    def fd(fun, h):
+        v = [fun(k * h) for k in (2, 1, -1, -2)]
+        return (-v[0] + 8 * v[1] - 8 * v[2] + v[3]) / (12 * h)
+
+    def F_at_D(dD):
+        g = orc.Grid(m, (D0 + dD, 1.0, 1.0))
+        Wk, rk = _converge(g, W0.copy(), chi, N, f, alpha, phic, ens)
+        return _free_energy(Wk, rk, chi)
+
+    def F_at_chi(dc):
+        Wk, rk = _converge(orc.Grid(m, (D0, 1.0, 1.0)), W0.copy(), chi + dc, N, f, alpha, phic, ens)
+        return _free_energy(Wk, rk, chi + dc)
+
+    out = {"stress": -dl[0], "stress_fd": fd(F_at_D, 0.02)}
+    if with_chi:
+        out["dchi"] = float(np.mean(r0["phi"][0] * r0["phi"][1]))
+        out["dchi_fd"] = fd(F_at_chi, 0.05)
+    return out
+
+
+def test_saddle_point_derivatives_match_free_energy_differences():
+    """dF/dD = -d lnQ/dD and dF/dchiN = <phiA phiB> at a saddle point (the envelope theorem),
+    against 4-point differences of FreeE's F with the fields re-converged at every point.
+    The discrete F is stationary in W only up to the contour discretisation, so the match
+    is O(ds^3): 7.5e-5 at N = 40 for 32 points, and it improves ~8x per halving of ds."""
+    orc.set_threads(1)  # 16 grid points: threads only add synchronisation
+    try:
+        r40 = _envelope(40, True)
+        r80 = _envelope(80, False)
+    finally:
+        orc.set_threads(os.cpu_count() or 1)
+    e40 = abs(r40["stress"] - r40["stress_fd"]) / abs(r40["stress_fd"])
+    c40 = abs(r40["dchi"] - r40["dchi_fd"]) / abs(r40["dchi_fd"])
+    assert e40 <= 3e-4 and c40 <= 3e-4, (r40, e40, c40)
+    e80 = abs(r80["stress"] - r80["stress_fd"]) / abs(r80["stress_fd"])
+    assert e80 <= e40 / 4, (e40, e80)
