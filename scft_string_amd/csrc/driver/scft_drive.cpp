@@ -63,6 +63,8 @@ bool has(const Params& p, const char* k) { return p.count(k) && !p.at(k).empty()
 struct Run {
     scft_params p{};
     int readwin = 1, maxiter = 4000, prerelax = 5000;
+    int relaxbox = 0;         // bit i: relax D_i to zero stress after the field solve (dostring = 0)
+    double stresstol = 1e-6;  // |dF/dD_i| bar of that relaxation
 };
 
 Run set_parameters(const Params& P)
@@ -103,6 +105,8 @@ Run set_parameters(const Params& P)
     geti("prerelax", r.prerelax);
     geti("device", p.device);
     geti("fe_fp64", p.fe_fp64);
+    geti("relaxbox", r.relaxbox);
+    get("stresstol", r.stresstol);
     if (has(P, "precision")) p.precision = P.at("precision").front() == 32 ? SCFT_FP32 : SCFT_FP64;
     if (has(P, "m")) for (size_t i = 0; i < P.at("m").size() && i < 3; ++i) p.m[i] = (int)P.at("m")[i];
     if (has(P, "D")) for (size_t i = 0; i < P.at("D").size() && i < 3; ++i) p.D[i] = P.at("D")[i];
@@ -306,6 +310,16 @@ int main(int argc, char** argv)
                 }
             }
             printf("Fields solved (0) (Time: %s)\n", elapsed(t0).c_str());
+        }
+        if (run.relaxbox & 7) {
+            if (R != 1 || p.dostring != 0)
+                throw scft_host::error(SCFT_ERR_INVALID, "relaxbox needs strn = 1 and dostring = 0");
+            double g[3] = {0, 0, 0};
+            int nbox = 0;
+            scft_host::check(ctx, scft_relax_box(ctx, run.relaxbox, run.stresstol, 1e-9, 5000, 50, p.D, g, &nbox),
+                             "relax_box");
+            printf("Box relaxed after %d boxes: D = %.10g %.10g %.10g, dF/dD = %.3e %.3e %.3e (Time: %s)\n", nbox,
+                   p.D[0], p.D[1], p.D[2], g[0], g[1], g[2], elapsed(t0).c_str());
         }
         free_energy();
         printf("Free energy caculated (procid = 0) (Time: %s)\n", elapsed(t0).c_str());
