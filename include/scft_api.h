@@ -141,6 +141,17 @@ scft_status scft_box_stress(scft_ctx *ctx, double *dFdD, double *dFdchi);
  * density.h:79-95, which the reference runs once at start-up). */
 scft_status scft_set_box(scft_ctx *ctx, const double D[3]);
 
+/* Independent solves of uneven size run together: one scft_string_iterate (iteration k[i],
+ * tolerance errTol) on each of the n contexts at once, each from its own host thread. Every
+ * context owns its streams, so the GPU runs their kernels concurrently, and small solves
+ * fill the CUs that one of them alone leaves idle. Results are the same as n calls in
+ * sequence. alf[i] and reps[i] are the per-context outputs of scft_string_iterate (alf and
+ * reps may be NULL). The contexts must be distinct. Returns SCFT_OK or the status of the
+ * first context (in index order) that failed; its message is in scft_last_error(ctxs[i]).
+ * No counterpart in the reference, which runs one replica per MPI rank. */
+scft_status scft_iterate_concurrent(scft_ctx *const *ctxs, int n, const int *k, double errTol,
+                                    double *const *alf, scft_iter_report *reps);
+
 /* Stress-free box relaxation of a single-replica context with dostring = 0. Starting
  * from box D (in/out), alternately (1) iterates the fields with the solve_field body
  * (scft_string_iterate, solve_w_plus tolerance 1e-3 as in scft.cu:1123) until errs <= errTol

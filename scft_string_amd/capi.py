@@ -20,7 +20,7 @@ EXPORTED = [
     "scft_props_batch_device", "scft_set_fields", "scft_get_fields", "scft_get_densities",
     "scft_get_tangent", "scft_solve_w_plus", "scft_string_iterate", "scft_free_energy",
     "scft_debug_qq_invariant", "scft_debug_time_props", "scft_fft_path", "scft_dlnq_dbox",
-    "scft_box_stress", "scft_set_box", "scft_relax_box",
+    "scft_box_stress", "scft_set_box", "scft_relax_box", "scft_iterate_concurrent",
 ]
 _STRING_RESULT = ("scft_last_error", "scft_fft_path")
 
@@ -81,6 +81,8 @@ def lib():
         L.scft_box_stress.argtypes = [vp, dp, dp]
         L.scft_set_box.argtypes = [vp, dp]
         L.scft_relax_box.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp, ip]
+        L.scft_iterate_concurrent.argtypes = [C.POINTER(vp), C.c_int, ip, C.c_double, C.POINTER(dp),
+                                              C.POINTER(IterReport)]
         for name in EXPORTED:
             if name not in _STRING_RESULT:
                 getattr(L, name).restype = C.c_int
@@ -202,19 +204,27 @@ class Context:
                                           k.ctypes.data_as(C.POINTER(C.c_int))))
         return k
 
-    def string_iterate(self, k, errTol=1e-3):
+    def _report(self):
         R = self.nrep
         arrs = {n: np.zeros(R) for n in ("errs", "errs3", "x", "xref", "dotWs", "dotWs2", "normDER", "normDEV")}
         steps = np.zeros(R, dtype=np.int32)
         alf = np.zeros(R)
         rep = IterReport(**{n: _dp(a) for n, a in arrs.items()}, steps=steps.ctypes.data_as(C.POINTER(C.c_int)))
-        self._chk(lib().scft_string_iterate(self._h, int(k), float(errTol), _dp(alf), C.byref(rep)))
+        return arrs, steps, alf, rep
+
+    @staticmethod
+    def _result(arrs, steps, alf, rep):
         out = dict(arrs)
         out["steps"] = steps
         out["alf"] = alf
         out["fitup"] = rep.fitup
         out["stepav"] = rep.stepav
         return out
+
+    def string_iterate(self, k, errTol=1e-3):
+        arrs, steps, alf, rep = self._report()
+        self._chk(lib().scft_string_iterate(self._h, int(k), float(errTol), _dp(alf), C.byref(rep)))
+        return self._result(arrs, steps, alf, rep)
 
     def free_energy(self):
         R = self.nrep
@@ -255,3 +265,22 @@ class Context:
                                        int(max_boxes), _dp(D), _dp(g), C.byref(nbox)))
         self.D = tuple(float(v) for v in D)
         return self.D, g, nbox.value
+
+
+def iterate_concurrent(ctxs, ks, errTol=1e-3):
+    """One string_iterate on each context at once (scft_iterate_concurrent). Returns the
+    per-context reports, as Context.string_iterate would."""
+    n = len(ctxs)
+    parts = [c._report() for c in ctxs]
+    hs = (C.c_void_p * n)(*[c._h for c in ctxs])
+    kk = (C.c_int * n)(*[int(k) for k in ks])
+    alfs = (C.POINTER(C.c_double) * n)(*[_dp(p[2]) for p in parts])
+    reps = (IterReport * n)(*[p[3] for p in parts])
+    rc = lib().scft_iterate_concurrent(hs, n, kk, float(errTol), alfs, reps)
+    if rc != SCFT_OK:
+        for c in ctxs:
+            msg = lib().scft_last_error(c._h).decode()
+            if msg:
+                raise ScftError(rc, msg)
+        raise ScftError(rc, "scft_iterate_concurrent failed")
+    return [Context._result(p[0], p[1], p[2], reps[i]) for i, p in enumerate(parts)]

@@ -151,9 +151,9 @@ std::string supported_grids(bool fp64)
 
 // ---------------------------------------------------------------- pencil (line) kernels
 template <typename T>
-static hipError_t pencil_setup(int m0, int m1, int m2)
+static hipError_t pencil_setup()
 {
-    const int lds = (int)pencil_lds_bytes<T>(m0, m1, m2);
+    const int lds = (int)pencil_lds_max<T>();  // shared by every context: never one grid's need
     const void* fns[] = {(const void*)k_pz<T>, (const void*)k_py<T, false>, (const void*)k_py<T, true>,
                          (const void*)k_px<T>};
     for (const void* f : fns) {
@@ -288,7 +288,7 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
         begin_ = &pencil_begin<T>;
         step_ = &pencil_step<T>;
         path_ = "pencil";
-        e = pencil_setup<T>(m[0], m[1], m[2]);
+        e = pencil_setup<T>();
     } else {
         char b[512];
         snprintf(b, sizeof b, "no %s propagator kernels for grid %dx%dx%d (supported: %s)",
@@ -421,7 +421,7 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
             (e = hipMalloc((void**)&Kf_, fld)) != hipSuccess || (e = hipMalloc((void**)&gpart_, pb)) != hipSuccess)
             return e;
         bytes_ += 3 * fld + pb;
-        const int lds = (int)pencil_lds_bytes<T>(m_[0], m_[1], m_[2]);
+        const int lds = (int)pencil_lds_max<T>();
         const void* fns[] = {(const void*)k_gz<T>, (const void*)k_gx<T>, (const void*)k_py<T, false>,
                              (const void*)k_py<T, true>};
         for (const void* f : fns)

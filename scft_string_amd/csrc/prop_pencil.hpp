@@ -348,7 +348,17 @@ inline bool pencil_supported(int m0, int m1, int m2)
     return true;
 }
 
-// dynamic LDS bytes of the three kernels (the largest is what setup reserves)
+// Upper bound of pencil_lds_bytes over every supported grid: a tile of at most PC_EMAX
+// elements plus one pad element per line (at most PC_LMAX lines) plus PC_LMAX twiddles.
+// The kernels' maximum dynamic LDS is a per-function attribute shared by every context in
+// the process, so setup reserves this bound rather than one grid's need.
+template <typename T>
+constexpr size_t pencil_lds_max()
+{
+    return (size_t)(PC_EMAX + 2 * PC_LMAX) * sizeof(cpx<T>);
+}
+
+// dynamic LDS bytes of the three kernels for one grid (what a launch passes)
 template <typename T>
 inline size_t pencil_lds_bytes(int m0, int m1, int m2)
 {

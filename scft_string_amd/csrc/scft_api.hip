@@ -10,6 +10,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "field_kernels.hpp"
@@ -653,6 +654,28 @@ scft_status scft_set_box(scft_ctx* c, const double D[3])
     if (e == hipSuccess && c->pe64_fe) e = c->pe64_fe->set_box(D);
     if (e != hipSuccess) return fail(c, SCFT_ERR_HIP, std::string("scft_set_box: ") + hipGetErrorString(e));
     for (int i = 0; i < 3; ++i) c->p.D[i] = D[i];
+    return SCFT_OK;
+}
+
+scft_status scft_iterate_concurrent(scft_ctx* const* ctxs, int n, const int* k, double errTol, double* const* alf,
+                                    scft_iter_report* reps)
+{
+    if (!ctxs || !k || n <= 0) return SCFT_ERR_INVALID;
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i]) return SCFT_ERR_INVALID;
+        for (int j = 0; j < i; ++j)
+            if (ctxs[j] == ctxs[i]) return fail(ctxs[i], SCFT_ERR_INVALID, "scft_iterate_concurrent: repeated context");
+    }
+    std::vector<scft_status> st(n, SCFT_OK);
+    std::vector<std::thread> th;
+    th.reserve(n);
+    for (int i = 0; i < n; ++i)
+        th.emplace_back([&, i]() {
+            st[i] = scft_string_iterate(ctxs[i], k[i], errTol, alf ? alf[i] : nullptr, reps ? &reps[i] : nullptr);
+        });
+    for (auto& t : th) t.join();
+    for (int i = 0; i < n; ++i)
+        if (st[i] != SCFT_OK) return st[i];
     return SCFT_OK;
 }
 
