@@ -177,6 +177,115 @@ def pmc_probe():
     print("pmc probe done", flush=True)
 
 
+def adjoint_report():
+    """Box derivative d lnQ/dD (adjoint of props) at 64^3, N = 100, chiN = 20, f = 0.5, FP32,
+    on a 1-replica and an 8-replica string (lamellar -> cylinder interpolants, seed 2024),
+    with D swept over 5.6 ... 6.4. Reports the mean time of props alone and props plus the
+    gradient (host-buffer API, so both include the PCIe copies of W and phi)."""
+    from scft_string_amd import Context
+
+    m, N = (64, 64, 64), 100
+    out = {"mode": "adjoint", "workload": "box gradient d lnQ/dD, 64^3, N=100, chiN=20, f=0.5, FP32",
+           "D_sweep": [5.6 + 0.1 * i for i in range(9)]}
+    for R in (1, 8):
+        W = initial_fields(m, (6.0,) * 3, max(R, 2))[:R]
+        tp = tg = 0.0
+        dmax = 0.0
+        with Context(m, (5.6,) * 3, N, 0.5, 0.1, 1, 0.8, 20.0, nrep=R, precision="fp32", dostring=0) as ctx:
+            ctx.props_batch(W)
+            ctx.dlnq_dbox(W)
+            for D in out["D_sweep"]:
+                ctx.set_box((D, D, D))
+                t = time.perf_counter()
+                ctx.props_batch(W)
+                tp += time.perf_counter() - t
+                t = time.perf_counter()
+                _, d = ctx.dlnq_dbox(W)
+                tg += time.perf_counter() - t
+                dmax = max(dmax, float(np.max(np.abs(d))))
+        n = len(out["D_sweep"])
+        out[f"replicas_{R}"] = {"props_ms": round(tp / n * 1e3, 2), "props_plus_gradient_ms": round(tg / n * 1e3, 2),
+                                "gradient_over_props": round((tg - tp) / tp, 3), "max_abs_dlnQ_dD": dmax}
+    print(json.dumps(out), flush=True)
+
+
+def uneven_report(K):
+    """Uneven solves on one GPU: a 24-replica string at 128^3, N = 160, FP32, grand canonical
+    (chiN = 22, f = 0.4, phidb = 0.85), lamellar -> gyroid-like ends, plus 12 independent
+    single-replica solves with m in {48^3, 64^3, 96^3} and N in {64, 100, 128} drawn with
+    seed 4 (same-(m, N) solves share a context as independent replicas). One round is one
+    solve_field iteration of every context. Concurrent rounds use scft_iterate_concurrent;
+    sequential rounds iterate the contexts one after another from identical fresh states."""
+    import gc
+
+    from scft_string_amd import Context, iterate_concurrent
+
+    chi, f, phidb, ens, alpha = 22.0, 0.4, 0.85, 2, 0.1
+    rng0 = np.random.default_rng(4)
+    solves = {}
+    for _ in range(12):
+        m = [(48,) * 3, (64,) * 3, (96,) * 3][rng0.integers(3)]
+        N = [64, 100, 128][rng0.integers(3)]
+        solves[(m, N)] = solves.get((m, N), 0) + 1
+
+    def grid(m):
+        return np.meshgrid(*[2 * np.pi * np.arange(m[i]) / m[i] for i in range(3)], indexing="ij")
+
+    def build(with_string):
+        rng = np.random.default_rng(7)
+        ctxs = []
+        if with_string:
+            m, R = (128,) * 3, 24
+            X, Y, Z = grid(m)
+            lam = 0.5 * np.cos(X)
+            gyr = 0.5 * (np.sin(X) * np.cos(Y) + np.sin(Y) * np.cos(Z) + np.sin(Z) * np.cos(X))
+            W = np.zeros((R, 2, X.size))
+            for i in range(R):
+                t = i / (R - 1)
+                W[i, 0] = ((1 - t) * lam + t * gyr).reshape(-1) + 0.01 * rng.standard_normal(X.size)
+            c = Context(m, (8.0,) * 3, 160, f, alpha, ens, phidb, chi, nrep=R, precision="fp32", fix=(1, 1))
+            c.set_fields(W)
+            ctxs.append(c)
+        for (m, N), R in sorted(solves.items()):
+            X = grid(m)[0]
+            W = np.zeros((R, 2, X.size))
+            for i in range(R):
+                W[i, 0] = (0.5 * np.cos(X)).reshape(-1) + 0.01 * rng.standard_normal(X.size)
+            c = Context(m, (m[0] / 16.0,) * 3, N, f, alpha, ens, phidb, chi, nrep=R, precision="fp32", dostring=0)
+            c.set_fields(W)
+            ctxs.append(c)
+        return ctxs
+
+    def run(with_string, concurrent):
+        ctxs = build(with_string)
+        if concurrent:
+            iterate_concurrent(ctxs, [0] * len(ctxs))
+        else:
+            for c in ctxs:
+                c.string_iterate(0)
+        t = time.perf_counter()
+        props = 0
+        for k in range(1, K + 1):
+            reps = iterate_concurrent(ctxs, [k] * len(ctxs)) if concurrent else [c.string_iterate(k) for c in ctxs]
+            props += sum(int(r["steps"].sum()) for r in reps)
+        dt = (time.perf_counter() - t) / K
+        for c in ctxs:
+            c.close()
+        gc.collect()
+        return dt, props // K
+
+    out = {"mode": "uneven", "rounds": K,
+           "independent_solves": {f"{m[0]}^3 N{N}": n for (m, N), n in sorted(solves.items())}}
+    for with_string in (False, True):
+        tc, pc = run(with_string, True)
+        ts, ps = run(with_string, False)
+        key = "string_24x128cube_N160_plus_12_solves" if with_string else "12_independent_solves"
+        out[key] = {"concurrent_ms_per_round": round(tc * 1e3, 1), "sequential_ms_per_round": round(ts * 1e3, 1),
+                    "props_per_round": [pc, ps], "speedup": round(ts / tc, 3)}
+        print(f"uneven: {key} done", file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,14 +297,24 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--pmc-probe", action="store_true")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="box-gradient timing: 64^3, N=100, FP32, D swept over 5.6-6.4, 1 and 8 replicas")
+    ap.add_argument("--uneven", action="store_true",
+                    help="24 x 128^3 N=160 string + 12 independent solves, concurrent vs sequential")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: all string replicas share it; "
-                                   "multi-GPU replica sharding is a design note only (DESIGN.md)",
+                                   "sharding replicas over GPUs is designed but not built",
                           "n_gpus": a.gpus}))
         sys.exit(2)
     if a.pmc_probe:
         pmc_probe()
+        return
+    if a.adjoint:
+        adjoint_report()
+        return
+    if a.uneven:
+        uneven_report(a.steps)
         return
 
     import torch
