@@ -304,29 +304,36 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT
     const size_t fo = (size_t)slot * (size_t)a.M + (size_t)x * NE;
     p2_load_tw<T, NY, NZ>(twz, twy, a, tid, NT);
 
-    // issue every global load of this plane up front: both k-space inputs and the factors
+    // Up to 64^2 planes every global load of the plane is issued up front (both k-space
+    // inputs and the factors). Larger planes run 1024 threads at <= 128 VGPRs, so they
+    // load Cf after the Ch inverse and the factors after both inverses instead of spilling.
+    constexpr bool EARLY = NE <= 4096;
     cpx<T> uh[P::IY2][P::B1], uf[P::IY2][P::B1];
     if (mode != PL_BEGIN) {
         P::inv_load(a.Ch + fo, uh, tid);
-        P::inv_load(a.Cf + fo, uf, tid);
+        if constexpr (EARLY) P::inv_load(a.Cf + fo, uf, tid);
     }
     T ea[IZ1][A2], eb[IZ1][A2];
+    auto load_factors = [&]() {
 #pragma unroll
-    for (int it = 0; it < IZ1; ++it) {
-        const int w = tid + it * NT;
-        if (P::z1_valid(w)) {
+        for (int it = 0; it < IZ1; ++it) {
+            const int w = tid + it * NT;
+            if (P::z1_valid(w)) {
 #pragma unroll
-            for (int a2 = 0; a2 < A2; ++a2) {
-                const int e = P::z1_elem(w, a2);
-                ea[it][a2] = a.e4A[fo + e];
-                eb[it][a2] = a.e4B[fo + e];
+                for (int a2 = 0; a2 < A2; ++a2) {
+                    const int e = P::z1_elem(w, a2);
+                    ea[it][a2] = a.e4A[fo + e];
+                    eb[it][a2] = a.e4B[fo + e];
+                }
             }
         }
-    }
+    };
+    if constexpr (EARLY) load_factors();
     cpx<T> q[IZ1][A2];
     int sout;
     __syncthreads();  // twiddles
     if (mode == PL_BEGIN) {
+        if constexpr (!EARLY) load_factors();
 #pragma unroll
         for (int it = 0; it < IZ1; ++it)
 #pragma unroll
@@ -336,7 +343,9 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT
         cpx<T> h1[IZ1][A2];
         P::inv_regs(uh, buf, twz, twy, q, tid);  // q holds qs0 for now
         __syncthreads();
+        if constexpr (!EARLY) P::inv_load(a.Cf + fo, uf, tid);
         P::inv_regs(uf, buf, twz, twy, h1, tid);
+        if constexpr (!EARLY) load_factors();
         const bool qa = s < a.n1, pb = s < a.n2;  // species of this step (density.h:136-146)
 #pragma unroll
         for (int it = 0; it < IZ1; ++it)
