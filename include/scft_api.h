@@ -58,6 +58,10 @@ typedef struct scft_params {
     int device;      /* HIP device ordinal */
     int fe_fp64;     /* FP32 contexts: 1 = the props that scft_free_energy evaluates F from
                         runs in FP64 (on the FP32-relaxed fields), 0 = in FP32 */
+    double phihA;    /* A homopolymer, which the reference declares but holds at zA = 0
+                        (density.h:352, 390-397, README.md:3). ens 1: its volume fraction
+                        (B homopolymer: 1 - phidb - phihA); ens 2: its activity zA. Length
+                        Nh, propagated by q1, so it needs Nh <= n1. 0 = the reference. */
 } scft_params;
 
 typedef struct scft_ctx scft_ctx;

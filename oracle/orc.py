@@ -74,6 +74,7 @@ def lib():
                                       dp, dp, dp, dp]
         L.orc_dlnq_dbox.argtypes = [vp, dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                     dp, dp]
+        L.orc_grid_set_phiha.argtypes = [vp, C.c_double]
         _LIB = L
     return _LIB
 
@@ -149,6 +150,10 @@ class Grid:
         if want_q:
             out["q"] = qs.reshape(N + 1, 2, M)
         return out
+
+    def set_phihA(self, phihA):
+        """A homopolymer for later calls (orc_grid_set_phiha): ens 1 volume fraction, ens 2 activity."""
+        lib().orc_grid_set_phiha(self._h, float(phihA))
 
     def dlnq_dbox(self, W, N, alpha, phic, f, ens):
         """Exact d lnQ / dD_i (i = x, y, z) of the discrete props at fixed W: (lnQ, (3,))."""

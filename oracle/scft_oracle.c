@@ -123,7 +123,10 @@ struct orc_grid {
     double *qs0, *qs1;      /* [M] */
     double *qq;             /* [2(N+1)M] propagator store */
     size_t qq_cap;
+    double phihA;           /* A homopolymer (0 = the reference, zA = 0) */
 };
+
+void orc_grid_set_phiha(orc_grid *g, double phihA) { g->phihA = phihA; }
 
 void orc_set_threads(int n)
 {
@@ -383,19 +386,24 @@ void orc_props(orc_grid *g, double *w, double *lnQ, double *phis, int Ns, double
         else        orc_step(g, g->e4A, g->e2A, g->k12, g->k6, Q2P(s), Q2P(s + 1));
     }
 
-    /* Q (density.h:377-386) */
+    /* Q (density.h:377-386); Q1 of the A homopolymer over q1 up to Nh */
     Q2 = sum_all(Q2P(Nh), M);
     QC = sum_all(Q1P(Ns), M);
+    Q1 = sum_all(Q1P(Nh), M);
     Q1 /= M; Q2 /= M; QC /= M;
 
-    /* ensemble (density.h:388-399) */
+    /* ensemble (density.h:388-399). With g->phihA = 0 this is the reference (zA = 0);
+     * otherwise ens 1 gives the A homopolymer the volume fraction phihA and the B
+     * homopolymer 1 - phic - phihA, and ens 2 gives it the activity zA = phihA. */
     if (ens == 1) {
-        *lnQ = phic * log(QC) + ((1.0 - phic) / alpha) * log(Q2);
-        zA = 0;
-        zB = (1.0 - phic) / (Q2 * alpha);
+        const double phihB = 1.0 - phic - g->phihA;
+        *lnQ = phic * log(QC) + (phihB / alpha) * log(Q2);
+        if (g->phihA != 0.0) *lnQ += (g->phihA / alpha) * log(Q1);
+        zA = g->phihA / (Q1 * alpha);
+        zB = phihB / (Q2 * alpha);
         zC = phic / QC;
     } else if (ens == 2) {
-        zA = 0;
+        zA = g->phihA;
         zB = 1.0;
         zC = phic;
         *lnQ = zA * Q1 + zB * Q2 + zC * QC;
@@ -411,12 +419,15 @@ void orc_props(orc_grid *g, double *w, double *lnQ, double *phis, int Ns, double
 #pragma omp parallel for schedule(static)
     for (int r = 0; r < M; r++) {
         /* rA_rB_init_Ns_homo (GPU_kernels.h:92-101) */
-        double hA = 0;
+        /* The reference starts rhA at 0 (GPU_kernels.h:96, "no A homopolymers"). The A
+         * homopolymer here gets the same end terms as the B one, 2 q1[Nh]. Both are
+         * multiplied by zA, which is 0 unless phihA is set. */
+        double hA = 2.0 * Q1P(Nh)[r];
         double hB = 2.0 * Q2P(Nh)[r];
         double cA = Q2P(Ns)[r] + Q1P(n1)[r] * Q2P(n2)[r];
         double cB = Q1P(Ns)[r] + Q1P(n1)[r] * Q2P(n2)[r];
         int s, wt;
-        /* homopolymer A: accumulated then multiplied by zA = 0 (density.h:405-407; A.9) */
+        /* homopolymer A (density.h:405-407): zA = 0 in the reference (A.9) */
         for (s = 1, wt = 4; s < Nh; s++, wt = 6 - wt) hA += wt * Q1P(s)[r] * Q1P(Nh - s)[r];
         for (s = 1, wt = 4; s < Nh; s++, wt = 6 - wt) hB += wt * Q2P(s)[r] * Q2P(Nh - s)[r];
         for (s = 1, wt = 4; s < n1; s++, wt = 6 - wt) cA += wt * Q1P(s)[r] * Q2P(Ns - s)[r];
@@ -795,7 +806,7 @@ void orc_dlnq_dbox(orc_grid *g, double *w, int Ns, double alpha, double phic, do
     const double ds = 1.0 / Ns;
     double Q[3], phis[5];
     orc_props(g, w, lnQ, phis, Ns, alpha, phic, f, ens, Q, NULL);
-    const double Q2 = Q[1], QC = Q[2];
+    const double Q1 = Q[0], Q2 = Q[1], QC = Q[2];
 #define Q1P(s) (g->qq + (size_t)2 * (s) * M)
 #define Q2P(s) (g->qq + ((size_t)2 * (s) + 1) * M)
     double *dk12[3], *dk6[3], *kc = malloc(sizeof(double) * Mk);
@@ -808,7 +819,7 @@ void orc_dlnq_dbox(orc_grid *g, double *w, int Ns, double alpha, double phic, do
             dk6[i][k] = g->k6[k] * kc[k] * ds / (3 * g->D[i]);
         }
     }
-    double dQC[3] = {0, 0, 0}, dQ2[3] = {0, 0, 0};
+    double dQC[3] = {0, 0, 0}, dQ2[3] = {0, 0, 0}, dQ1[3] = {0, 0, 0};
     for (int s = 0; s < Ns; s++) {
         const int isA = s < n1; /* species of chain 1 on step s (density.h:136-146) */
         grad_term(g, isA ? g->e4A : g->e4B, isA ? g->e2A : g->e2B, dk12, dk6, Q2P(Ns - 1 - s),
@@ -819,9 +830,19 @@ void orc_dlnq_dbox(orc_grid *g, double *w, int Ns, double alpha, double phic, do
         grad_term(g, isB ? g->e4B : g->e4A, isB ? g->e2B : g->e2A, dk12, dk6, Q2P(Nh - 1 - t),
                   Q2P(t), dQ2);
     }
+    if (g->phihA != 0.0)
+        for (int t = 0; t < Nh; t++) {
+            const int isA = t < n1; /* the A homopolymer's steps: q1 up to Nh */
+            grad_term(g, isA ? g->e4A : g->e4B, isA ? g->e2A : g->e2B, dk12, dk6, Q1P(Nh - 1 - t),
+                      Q1P(t), dQ1);
+        }
     for (int i = 0; i < 3; i++) {
-        if (ens == 1) dlnQ[i] = phic * dQC[i] / QC + ((1.0 - phic) / alpha) * dQ2[i] / Q2;
-        else dlnQ[i] = dQ2[i] + phic * dQC[i];
+        if (ens == 1) {
+            dlnQ[i] = phic * dQC[i] / QC + ((1.0 - phic - g->phihA) / alpha) * dQ2[i] / Q2;
+            if (g->phihA != 0.0) dlnQ[i] += (g->phihA / alpha) * dQ1[i] / Q1;
+        } else {
+            dlnQ[i] = dQ2[i] + phic * dQC[i] + g->phihA * dQ1[i];
+        }
         free(dk12[i]);
         free(dk6[i]);
     }

@@ -108,6 +108,13 @@ double orc_free_energy(orc_grid *g, const orc_run *run, double **W, double **DEV
                        const int *doiis, double *alf, double *FEs0, double *FEs1, double *FEs,
                        double *phctot);
 
+/* A homopolymer for every later orc_props / orc_dlnq_dbox on this grid (default 0, the
+ * reference's zA = 0, density.h:352, 390-397). ens 1: volume fraction (B homopolymer
+ * 1 - phic - phihA); ens 2: activity zA. It uses q1 up to Nh (needs Nh <= n1). Pinned by
+ * the A <-> B mirror symmetry: negating W- and swapping f <-> 1 - f maps it onto the
+ * reference's B homopolymer. */
+void orc_grid_set_phiha(orc_grid *g, double phihA);
+
 /* Exact derivative of the discrete lnQ of density::props (density.h:344-399) with respect
  * to the box lengths D[0..2] at fixed W, from the stored q1, q2 as adjoints. w as in
  * orc_props (phi out). dlnQ[3] out. Pinned by central differences of orc_props. */

@@ -105,6 +105,7 @@ Run set_parameters(const Params& P)
     geti("prerelax", r.prerelax);
     geti("device", p.device);
     geti("fe_fp64", p.fe_fp64);
+    get("phihA", p.phihA);
     geti("relaxbox", r.relaxbox);
     get("stresstol", r.stresstol);
     if (has(P, "precision")) p.precision = P.at("precision").front() == 32 ? SCFT_FP32 : SCFT_FP64;

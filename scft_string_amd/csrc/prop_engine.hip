@@ -308,7 +308,7 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
         !alloc((void**)&e4B_, sizeof(T) * (size_t)M_ * nbmax) ||
         !alloc((void**)&qs_, sizeof(cpx<T>) * (size_t)M_ * (size_t)(N + 1) * nbmax) ||
         !alloc((void**)&part_, sizeof(double) * (size_t)nblk_ * 5 * nbmax) ||
-        !alloc((void**)&cAB_, sizeof(double) * 2 * nbmax) || !alloc((void**)&scal_, sizeof(double) * 8 * nbmax))
+        !alloc((void**)&cAB_, sizeof(double) * 2 * nbmax) || !alloc((void**)&scal_, sizeof(double) * SCAL_N * nbmax))
         return "device allocation failed (propagator buffers)";
     for (int i = 0; i < 3; ++i) {
         e = upload_twiddles<T>(&tw_[i], m[i]);
@@ -416,7 +416,7 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
     hipError_t e;
     if (!gpart_) {
         const size_t fld = sizeof(cpx<T>) * (size_t)M_ * nbmax_;
-        const size_t pb = sizeof(double) * (size_t)nbmax_ * 2 * nblk_ * 3;
+        const size_t pb = sizeof(double) * (size_t)nbmax_ * 3 * nblk_ * 3;
         if ((e = hipMalloc((void**)&K0_, fld)) != hipSuccess || (e = hipMalloc((void**)&K1_, fld)) != hipSuccess ||
             (e = hipMalloc((void**)&Kf_, fld)) != hipSuccess || (e = hipMalloc((void**)&gpart_, pb)) != hipSuccess)
             return e;
@@ -428,7 +428,7 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
             if ((e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds)) != hipSuccess) return e;
     }
     const size_t np = (size_t)nb * nblk_ * 3;  // partials per chain
-    if ((e = hipMemsetAsync(gpart_, 0, sizeof(double) * 2 * np, st)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(gpart_, 0, sizeof(double) * 3 * np, st)) != hipSuccess) return e;
 
     const PropArgs<T> a = args(nb, cc);
     const PencilTiles p = pencil_tiles(m_[0], m_[1], m_[2]);
@@ -438,16 +438,18 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
     const long long lines = (long long)m_[0] * m_[1];
     const dim3 gzg((unsigned)((lines + p.nlz - 1) / p.nlz), nb);
     const int ntx = (m_[2] + p.tx - 1) / p.tx;
-    for (int chain = 0; chain < 2; ++chain) {
-        const int ns = chain == 0 ? cc.N : cc.Nh;
+    for (int chain = 0; chain < 3; ++chain) {
+        // chain 2 (the A homopolymer's Q1) only when it is present
+        const int ns = chain == 0 ? cc.N : (chain == 2 && cc.phihA == 0.0 ? 0 : cc.Nh);
         double* pc = gpart_ + (size_t)chain * np;
         for (int s = 0; s < ns; ++s) {
             // chain 0: b = q1[s], a = q2[N-1-s], species of chain 1 on step s (density.h:136-146)
             // chain 1: b = q2[t], a = q2[Nh-1-t], species of chain 2 on step t
+            // chain 2: b = q1[t], a = q1[Nh-1-t], species of chain 1 on step t (A for Nh <= n1)
             const int sa = chain == 0 ? cc.N - 1 - s : cc.Nh - 1 - s;
-            const int pbk = chain == 0 ? 0 : 1;
-            const int spec = chain == 0 ? (s < cc.n1 ? 0 : 1) : (s < cc.n2 ? 1 : 0);
-            hipLaunchKernelGGL((k_gz<T>), gzg, dim3(PC_NT), lds, st, a, (int)GZ_INIT, s, pbk, sa, 1, spec, p.nlz);
+            const int pbk = chain == 1 ? 1 : 0, pak = chain == 2 ? 0 : 1;
+            const int spec = chain == 1 ? (s < cc.n2 ? 1 : 0) : (s < cc.n1 ? 0 : 1);
+            hipLaunchKernelGGL((k_gz<T>), gzg, dim3(PC_NT), lds, st, a, (int)GZ_INIT, s, pbk, sa, pak, spec, p.nlz);
             pencil_py<T, false>(a, nb, 2 * nb, p, lds, st);
             hipLaunchKernelGGL((k_gx<T>), dim3(m_[1] * ntx, 2 * nb), dim3(PC_NT), lds, st, a, (int)GX_SPLIT, nb,
                                p.tx, K0_, Kf_);
@@ -461,17 +463,17 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
         }
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    std::vector<double> h(2 * np);
-    if ((e = hipMemcpyAsync(h.data(), gpart_, sizeof(double) * 2 * np, hipMemcpyDeviceToHost, st)) != hipSuccess)
+    std::vector<double> h(3 * np);
+    if ((e = hipMemcpyAsync(h.data(), gpart_, sizeof(double) * 3 * np, hipMemcpyDeviceToHost, st)) != hipSuccess)
         return e;
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
     const double scale = 1.0 / ((double)M_ * (double)M_);
     for (int slot = 0; slot < nb; ++slot)
-        for (int chain = 0; chain < 2; ++chain)
+        for (int chain = 0; chain < 3; ++chain)
             for (int i = 0; i < 3; ++i) {
                 double v = 0;
                 for (int b = 0; b < nblk_; ++b) v += h[(size_t)chain * np + ((size_t)slot * nblk_ + b) * 3 + i];
-                out[(size_t)slot * 6 + chain * 3 + i] = v * scale;
+                out[(size_t)slot * 9 + chain * 3 + i] = v * scale;
             }
     return hipSuccess;
 }

@@ -49,12 +49,13 @@ public:
                    const int* d_slot2rep, int nb, const ChainConst& cc, double* d_phis,
                    hipStream_t st);
 
-    // device [nbmax][8]: lnQ, sC, sH, QC, Q2, Q1, QC', Q2'
+    // device [nbmax][SCAL_N]: lnQ, sC, sH, QC, Q2, Q1, QC', Q2', sHA, Q1'
     const double* scal() const { return scal_; }
     // qq[s] = <q1'[s] q2'[N-s]> for s = 0..N of one slot (host array of N+1)
     hipError_t qq_invariant(int slot, double* qq, hipStream_t st);
     // Box derivatives of the factored partition functions of the nb slots of the last run()
-    // (same cc): out[slot][0..2] = dQC'/dD_i, out[slot][3..5] = dQ2'/dD_i (host array).
+    // (same cc): out[slot][0..2] = dQC'/dD_i, out[slot][3..5] = dQ2'/dD_i, out[slot][6..8] =
+    // dQ1'/dD_i (zero when cc.phihA == 0) (host array).
     // hipErrorNotSupported when an axis does not factor into 2, 3, 5 or exceeds 1024.
     hipError_t box_gradient(int nb, const ChainConst& cc, double* out, hipStream_t st);
     const cpx<T>* qstore() const { return qs_; }

@@ -362,7 +362,7 @@ __global__ void __launch_bounds__(RED_NT) k_factors(const double* __restrict__ W
     }
 }
 
-// partial sums of q1'[N] and q2'[Nh] (density.h:377-381)
+// partial sums of q1'[N], q2'[Nh] (density.h:377-381) and q1'[Nh] (Q1 of the A homopolymer)
 template <typename T>
 __global__ void __launch_bounds__(RED_NT) k_qsum(const cpx<T>* __restrict__ qs, long long M, int N, int Nh,
                                                  double* __restrict__ part)
@@ -371,16 +371,20 @@ __global__ void __launch_bounds__(RED_NT) k_qsum(const cpx<T>* __restrict__ qs, 
     const int slot = blockIdx.y;
     const cpx<T>* qN = qs + ((size_t)slot * (N + 1) + N) * (size_t)M;
     const cpx<T>* qH = qs + ((size_t)slot * (N + 1) + Nh) * (size_t)M;
-    double sc = 0, sh2 = 0;
+    double sc = 0, sh2 = 0, sh1 = 0;
     for (long long r = (long long)blockIdx.x * RED_NT + threadIdx.x; r < M; r += (long long)gridDim.x * RED_NT) {
         sc += (double)qN[r].x;
         sh2 += (double)qH[r].y;
+        sh1 += (double)qH[r].x;
     }
     sc = block_sum(sc, sh);
     sh2 = block_sum(sh2, sh);
+    sh1 = block_sum(sh1, sh);
     if (threadIdx.x == 0) {
-        part[((size_t)slot * gridDim.x + blockIdx.x) * 2 + 0] = sc;
-        part[((size_t)slot * gridDim.x + blockIdx.x) * 2 + 1] = sh2;
+        double* o = part + ((size_t)slot * gridDim.x + blockIdx.x) * 3;
+        o[0] = sc;
+        o[1] = sh2;
+        o[2] = sh1;
     }
 }
 
@@ -390,28 +394,35 @@ __global__ void __launch_bounds__(RED_NT) k_scalars(const double* __restrict__ p
 {
     __shared__ double sh[RED_NT];
     const int slot = blockIdx.x;
-    const double* p = part + (size_t)slot * nblk * 2;
-    const double sQC = sum_partials(p, nblk, 2, 0, sh);
-    const double sQ2 = sum_partials(p, nblk, 2, 1, sh);
+    const double* p = part + (size_t)slot * nblk * 3;
+    const double sQC = sum_partials(p, nblk, 3, 0, sh);
+    const double sQ2 = sum_partials(p, nblk, 3, 1, sh);
+    const double sQ1 = sum_partials(p, nblk, 3, 2, sh);
     if (threadIdx.x != 0) return;
     const double cA = cAB[2 * slot], cB = cAB[2 * slot + 1];
-    const double QCp = sQC / (double)M, Q2p = sQ2 / (double)M;
+    const double QCp = sQC / (double)M, Q2p = sQ2 / (double)M, Q1p = sQ1 / (double)M;
     const double LC = (cc.n1 * cA + cc.n2 * cB) * cc.ds;  // q1(N) = q1'(N) exp(-LC)
     const double LH = cc.Nh * cB * cc.ds;                  // q2(Nh) = q2'(Nh) exp(-LH)
-    const double lQC = log(QCp) - LC, lQ2 = log(Q2p) - LH;
-    const double QC = exp(lQC), Q2 = exp(lQ2), Q1 = 0.0;
-    double lnQ, sC, sH;
+    const double LA = cc.Nh * cA * cc.ds;                  // q1(Nh) = q1'(Nh) exp(-LA), Nh <= n1
+    const double lQC = log(QCp) - LC, lQ2 = log(Q2p) - LH, lQ1 = log(Q1p) - LA;
+    const double QC = exp(lQC), Q2 = exp(lQ2), Q1 = exp(lQ1);
+    double lnQ, sC, sH, sHA;
     if (cc.ens == 1) {
-        lnQ = cc.phic * lQC + ((1.0 - cc.phic) / cc.alpha) * lQ2;
-        sC = cc.phic / QCp * cc.ds / 3.0;                         // zC e^{-LC} = phic / QC'
-        sH = (1.0 - cc.phic) / (Q2p * cc.alpha) * cc.ds / 3.0;    // zB e^{-LH}
+        const double phihB = 1.0 - cc.phic - cc.phihA;
+        lnQ = cc.phic * lQC + (phihB / cc.alpha) * lQ2;
+        if (cc.phihA != 0.0) lnQ += (cc.phihA / cc.alpha) * lQ1;
+        sC = cc.phic / QCp * cc.ds / 3.0;                    // zC e^{-LC} = phic / QC'
+        sH = phihB / (Q2p * cc.alpha) * cc.ds / 3.0;         // zB e^{-LH}
+        sHA = cc.phihA / (Q1p * cc.alpha) * cc.ds / 3.0;     // zA e^{-LA}
     } else {
-        lnQ = 0.0 * Q1 + 1.0 * Q2 + cc.phic * QC;
+        lnQ = cc.phihA * Q1 + 1.0 * Q2 + cc.phic * QC;
         sC = cc.phic * exp(-LC) * cc.ds / 3.0;
         sH = exp(-LH) * cc.ds / 3.0;
+        sHA = cc.phihA * exp(-LA) * cc.ds / 3.0;
     }
-    double* o = scal + (size_t)slot * 8;
+    double* o = scal + (size_t)slot * SCAL_N;
     o[0] = lnQ; o[1] = sC; o[2] = sH; o[3] = QC; o[4] = Q2; o[5] = Q1; o[6] = QCp; o[7] = Q2p;
+    o[8] = sHA; o[9] = Q1p;
 }
 
 // Simpson weights of density.h:401-422: end terms 1; interior 4,2,4,... from the first
@@ -430,9 +441,10 @@ __global__ void __launch_bounds__(RED_NT) k_density(const cpx<T>* __restrict__ q
     const int slot = blockIdx.y;
     const int N = cc.N, n1 = cc.n1, Nh = cc.Nh;
     const cpx<T>* q = qs + (size_t)slot * (N + 1) * (size_t)M;
-    const double sC = scal[(size_t)slot * 8 + 1], sH = scal[(size_t)slot * 8 + 2];
+    const double sC = scal[(size_t)slot * SCAL_N + 1], sH = scal[(size_t)slot * SCAL_N + 2],
+                 sHA = scal[(size_t)slot * SCAL_N + 8];
     double* ph = phi + (size_t)slot2rep[slot] * (size_t)phistride;
-    double a0 = 0, a1 = 0, a2 = 0, a4 = 0;
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
     for (long long r = (long long)blockIdx.x * RED_NT + threadIdx.x; r < M; r += (long long)gridDim.x * RED_NT) {
         double cA = 0, cB = 0;
         // copolymer terms q1[s] q2[N-s], s = 0..N, read in pairs (s, N-s)
@@ -458,15 +470,19 @@ __global__ void __launch_bounds__(RED_NT) k_density(const cpx<T>* __restrict__ q
                 else cB += v;
             }
         }
-        // homopolymer B: 2 q2[Nh] + sum_{s=1}^{Nh-1} w_s q2[s] q2[Nh-s]
-        double hB = 2.0 * (double)q[(size_t)Nh * M + r].y;
+        // homopolymers: B = 2 q2[Nh] + sum_{s=1}^{Nh-1} w_s q2[s] q2[Nh-s] (density.h:409-412),
+        // A the same over q1 (density.h:404-407), from the same slots' real parts
+        const cpx<T> zh = q[(size_t)Nh * M + r];
+        double hB = 2.0 * (double)zh.y, hA = 2.0 * (double)zh.x;
         for (int s = 1; 2 * s <= Nh; ++s) {
             const int t = Nh - s;
             if (t < 1) break;
-            const double v = (double)q[(size_t)s * M + r].y * (double)q[(size_t)t * M + r].y;
-            hB += (t != s) ? (wt_odd_first(s) + wt_odd_first(t)) * v : wt_odd_first(s) * v;
+            const cpx<T> zs = q[(size_t)s * M + r], zt = q[(size_t)t * M + r];
+            const double w = (t != s) ? wt_odd_first(s) + wt_odd_first(t) : wt_odd_first(s);
+            hB += w * ((double)zs.y * (double)zt.y);
+            hA += w * ((double)zs.x * (double)zt.x);
         }
-        const double rCA = cA * sC, rCB = cB * sC, rhB = hB * sH, rhA = 0.0;
+        const double rCA = cA * sC, rCB = cB * sC, rhB = hB * sH, rhA = hA * sHA;
         const double rA = rhA + rCA, rB = rhB + rCB;
         ph[r] = rA;
         ph[r + M] = rB;
@@ -475,15 +491,17 @@ __global__ void __launch_bounds__(RED_NT) k_density(const cpx<T>* __restrict__ q
         a0 += rCA + rCB;
         a1 += rA;
         a2 += rB;
+        a3 += rhA;
         a4 += rhB;
     }
     a0 = block_sum(a0, sh);
     a1 = block_sum(a1, sh);
     a2 = block_sum(a2, sh);
+    a3 = block_sum(a3, sh);
     a4 = block_sum(a4, sh);
     if (threadIdx.x == 0) {
         double* o = part + ((size_t)slot * gridDim.x + blockIdx.x) * 5;
-        o[0] = a0; o[1] = a1; o[2] = a2; o[3] = 0.0; o[4] = a4;
+        o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3; o[4] = a4;
     }
 }
 

@@ -26,11 +26,19 @@ struct PropArgs {
     T invM;
 };
 
-// Per-slot scalars (density.h:376-399) in FP64 from the factored sums.
-// scal[slot] = {lnQ, sC, sH, QC, Q2, Q1}; sC, sH multiply the Simpson sums directly.
+// Per-slot scalars (density.h:376-399) in FP64 from the factored sums:
+// scal[slot][SCAL_N] = {lnQ, sC, sH, QC, Q2, Q1, QC', Q2', sHA, Q1'}. sC, sH, sHA multiply
+// the copolymer, B-homopolymer and A-homopolymer Simpson sums directly; primes are the
+// partition functions of the mean-field-factored propagators.
+constexpr int SCAL_N = 10;
+
+// phihA: the A homopolymer the reference declares but leaves at zA = 0 (density.h:352,
+// 390-397). Canonical: its volume fraction, the B homopolymer taking 1 - phic - phihA;
+// grand canonical: its activity zA. It has the B homopolymer's length Nh and uses the
+// copolymer's A-end propagator q1 up to Nh, so Nh <= n1 (density.h:405-407).
 struct ChainConst {
     int N, n1, n2, Nh, ens;
-    double alpha, phic, ds;
+    double alpha, phic, ds, phihA;
 };
 
 // Boltzmann-factor arithmetic per precision. a = (w - <w>) ds / 4 is the exponent of the

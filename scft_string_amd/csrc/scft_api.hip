@@ -18,6 +18,7 @@
 
 using scft::ChainConst;
 using scft::PropEngine;
+using scft::SCAL_N;
 
 struct scft_ctx {
     scft_params p{};
@@ -41,7 +42,7 @@ struct scft_ctx {
     double *dsm = nullptr, *dlam = nullptr, *dx = nullptr, *dxref = nullptr, *dcproj = nullptr, *dphisv = nullptr;
     int *dactive = nullptr, *dslot = nullptr;
     // pinned host staging
-    double* hsm = nullptr;  // [R*8]
+    double* hsm = nullptr;  // [R*SCAL_N]
     int* hslot = nullptr;   // [R]
     double* hsmall = nullptr;  // [4R] x, xref, lam, cproj
     int* hact = nullptr;    // [R]
@@ -84,13 +85,13 @@ scft_status run_props(scft_ctx* c, const double* dW, long long wstride, double* 
     if (e != hipSuccess) return fail(c, SCFT_ERR_HIP, std::string("propagator launch: ") + hipGetErrorString(e));
     if (lnQ_host || phis_host) {
         const double* scal = use32 ? c->pe32->scal() : p64->scal();
-        HIPCHK(c, hipMemcpyAsync(c->hsm, scal, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync(c->hsm, scal, sizeof(double) * SCAL_N * nb, hipMemcpyDeviceToHost, c->st));
         std::vector<double> ph(5 * (size_t)nb);
         if (phis_host)
             HIPCHK(c, hipMemcpyAsync(ph.data(), c->dphisv, sizeof(double) * 5 * nb, hipMemcpyDeviceToHost, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
         for (int i = 0; i < nb; ++i) {
-            if (lnQ_host) lnQ_host[i] = c->hsm[8 * i + 0];
+            if (lnQ_host) lnQ_host[i] = c->hsm[SCAL_N * i + 0];
             if (phis_host)
                 for (int j = 0; j < 5; ++j) phis_host[5 * i + j] = ph[5 * i + j];
         }
@@ -183,7 +184,11 @@ scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
     c->cc.alpha = p->alpha;
     c->cc.phic = p->phidb;
     c->cc.ds = c->ds;
+    c->cc.phihA = p->phihA;
     if (c->cc.n1 <= 0 || c->cc.n1 >= p->N || c->cc.Nh < 0 || c->cc.Nh > c->cc.n2) return SCFT_ERR_INVALID;
+    // the A homopolymer reuses q1 up to Nh, which is all-A only for Nh <= n1
+    if (p->phihA < 0 || (p->phihA > 0 && c->cc.Nh > c->cc.n1)) return SCFT_ERR_INVALID;
+    if (p->ens == 1 && p->phidb + p->phihA > 1.0) return SCFT_ERR_INVALID;
 
     if (hipSetDevice(p->device) != hipSuccess) return SCFT_ERR_HIP;
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return SCFT_ERR_HIP;
@@ -211,7 +216,7 @@ scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
               hipMalloc((void**)&c->dWs, sizeof(double) * 2 * M * R) == hipSuccess &&
               hipMalloc((void**)&c->dphis_rows, sizeof(double) * 4 * M * R) == hipSuccess &&
               hipMalloc((void**)&c->dpart, sizeof(double) * (size_t)c->nblk * 4 * R) == hipSuccess &&
-              hipMalloc((void**)&c->dsm, sizeof(double) * 8 * R) == hipSuccess &&
+              hipMalloc((void**)&c->dsm, sizeof(double) * SCAL_N * R) == hipSuccess &&
               hipMalloc((void**)&c->dlam, sizeof(double) * R) == hipSuccess &&
               hipMalloc((void**)&c->dx, sizeof(double) * R) == hipSuccess &&
               hipMalloc((void**)&c->dxref, sizeof(double) * R) == hipSuccess &&
@@ -219,7 +224,7 @@ scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
               hipMalloc((void**)&c->dphisv, sizeof(double) * 5 * R) == hipSuccess &&
               hipMalloc((void**)&c->dactive, sizeof(int) * R) == hipSuccess &&
               hipMalloc((void**)&c->dslot, sizeof(int) * R) == hipSuccess &&
-              hipHostMalloc((void**)&c->hsm, sizeof(double) * 8 * R, hipHostMallocDefault) == hipSuccess &&
+              hipHostMalloc((void**)&c->hsm, sizeof(double) * SCAL_N * R, hipHostMallocDefault) == hipSuccess &&
               hipHostMalloc((void**)&c->hslot, sizeof(int) * R, hipHostMallocDefault) == hipSuccess &&
               hipHostMalloc((void**)&c->hsmall, sizeof(double) * 4 * R, hipHostMallocDefault) == hipSuccess &&
               hipHostMalloc((void**)&c->hact, sizeof(int) * R, hipHostMallocDefault) == hipSuccess;
@@ -578,19 +583,23 @@ namespace {
 template <typename T>
 scft_status dlnq_from_engine(scft_ctx* c, PropEngine<T>* pe, int nb, double* dl)
 {
-    std::vector<double> d(6 * (size_t)nb), sc(8 * (size_t)nb);
+    std::vector<double> d(9 * (size_t)nb), sc(SCAL_N * (size_t)nb);
     hipError_t e = pe->box_gradient(nb, c->cc, d.data(), c->st);
     if (e == hipErrorNotSupported)
         return fail(c, SCFT_ERR_UNSUPPORTED, "box gradient: every axis must factor into 2, 3, 5 and be at most 1024");
     if (e != hipSuccess) return fail(c, SCFT_ERR_HIP, std::string("box gradient: ") + hipGetErrorString(e));
-    HIPCHK(c, hipMemcpyAsync(sc.data(), pe->scal(), sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(sc.data(), pe->scal(), sizeof(double) * SCAL_N * nb, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
-    const double phic = c->cc.phic, alpha = c->cc.alpha;
+    const double phic = c->cc.phic, alpha = c->cc.alpha, phihA = c->cc.phihA, phihB = 1.0 - phic - phihA;
     for (int j = 0; j < nb; ++j) {
-        const double QC = sc[8 * j + 3], Q2 = sc[8 * j + 4], QCp = sc[8 * j + 6], Q2p = sc[8 * j + 7];
+        const double* s = &sc[(size_t)SCAL_N * j];
+        const double QC = s[3], Q2 = s[4], Q1 = s[5], QCp = s[6], Q2p = s[7], Q1p = s[9];
         for (int i = 0; i < 3; ++i) {
-            const double rc = d[6 * j + i] / QCp, r2 = d[6 * j + 3 + i] / Q2p;  // d ln QC/dD_i, d ln Q2/dD_i
-            dl[3 * j + i] = c->cc.ens == 1 ? phic * rc + ((1.0 - phic) / alpha) * r2 : Q2 * r2 + phic * QC * rc;
+            // d ln QC/dD_i, d ln Q2/dD_i, d ln Q1/dD_i
+            const double rc = d[9 * j + i] / QCp, r2 = d[9 * j + 3 + i] / Q2p, r1 = d[9 * j + 6 + i] / Q1p;
+            double v = c->cc.ens == 1 ? phic * rc + (phihB / alpha) * r2 : Q2 * r2 + phic * QC * rc;
+            if (phihA != 0.0) v += c->cc.ens == 1 ? (phihA / alpha) * r1 : phihA * Q1 * r1;
+            dl[3 * j + i] = v;
         }
     }
     return SCFT_OK;
