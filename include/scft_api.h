@@ -62,6 +62,14 @@ typedef struct scft_params {
                         (density.h:352, 390-397, README.md:3). ens 1: its volume fraction
                         (B homopolymer: 1 - phidb - phihA); ens 2: its activity zA. Length
                         Nh, propagated by q1, so it needs Nh <= n1. 0 = the reference. */
+    int anderson;            /* dostring = 0 only: Anderson mixing over this many histories (at
+                                most 40) in scft_string_iterate, one props per iteration. The
+                                reference keeps only its arrays (scft.cu:30-31, 1029-1031).
+                                0 = the reference's mixing. */
+    double anderson_lambda;  /* its mixing fraction (<= 0: 0.2) */
+    double anderson_start;   /* simple mixing (lambda 0.05) until errs < this (<= 0: 1e-2;
+                                started much earlier, Anderson can land on another saddle
+                                point, e.g. the disordered one) */
 } scft_params;
 
 typedef struct scft_ctx scft_ctx;

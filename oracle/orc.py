@@ -75,6 +75,13 @@ def lib():
         L.orc_dlnq_dbox.argtypes = [vp, dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                     dp, dp]
         L.orc_grid_set_phiha.argtypes = [vp, C.c_double]
+        L.orc_anderson_create.restype = vp
+        L.orc_anderson_create.argtypes = [C.c_int, C.c_int]
+        L.orc_anderson_destroy.argtypes = [vp]
+        L.orc_anderson_iter.restype = C.c_double
+        L.orc_anderson_iter.argtypes = [vp, dp, vp, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
+                                        C.c_int, C.c_double, C.c_double]
+        L.orc_small_solve.argtypes = [C.c_int, dp, dp, dp]
         _LIB = L
     return _LIB
 
@@ -241,3 +248,28 @@ class StringState:
                                    self.doiis.ctypes.data_as(C.POINTER(C.c_int)), _dp(self.alf),
                                    _dp(FEs0), _dp(FEs1), _dp(FEs), _dp(ph))
         return {"FE": FE, "FEs0": FEs0, "FEs1": FEs1, "FEs": FEs, "phctot": ph, "alf": self.alf.copy()}
+
+
+class Anderson:
+    """Anderson mixing state of one replica (orc_anderson_*). iterate() updates W (2, M) in place."""
+
+    def __init__(self, grid, H, chi, f, N, alpha, phic, ens, lam, start):
+        self.g = grid
+        self.args = (float(chi), float(f), int(N), float(alpha), float(phic), int(ens), float(lam), float(start))
+        self._h = lib().orc_anderson_create(grid.M, int(H))
+
+    def iterate(self, W):
+        M = self.g.M
+        row = np.zeros(7 * M)
+        row[: 2 * M] = W.reshape(-1)
+        errs = lib().orc_anderson_iter(self.g._h, _dp(row), self._h, *self.args)
+        W.reshape(-1)[:] = row[: 2 * M]
+        return errs
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().orc_anderson_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

@@ -850,3 +850,131 @@ void orc_dlnq_dbox(orc_grid *g, double *w, int Ns, double alpha, double phic, do
 #undef Q1P
 #undef Q2P
 }
+
+/* ------------------------------------------------------------------------------------ */
+/* F3: Anderson mixing, an opt-in mode for dostring = 0 (the reference keeps only its    */
+/* arrays: DIM, DDEV, WIN, scft.cu:30-31, 1029-1031)                                      */
+/* ------------------------------------------------------------------------------------ */
+struct orc_anderson {
+    int M2, H, count, head, started;
+    double *Wr, *Dr; /* [(H+1)][2M] rings */
+    double *G;       /* [(H+1)^2] sums of D_i D_j over 2M */
+};
+
+orc_anderson *orc_anderson_create(int M, int H)
+{
+    orc_anderson *a = (orc_anderson *)calloc(1, sizeof(orc_anderson));
+    a->M2 = 2 * M;
+    a->H = H;
+    a->Wr = (double *)calloc((size_t)(H + 1) * a->M2, sizeof(double));
+    a->Dr = (double *)calloc((size_t)(H + 1) * a->M2, sizeof(double));
+    a->G = (double *)calloc((size_t)(H + 1) * (H + 1), sizeof(double));
+    return a;
+}
+
+void orc_anderson_destroy(orc_anderson *a)
+{
+    if (!a) return;
+    free(a->Wr); free(a->Dr); free(a->G); free(a);
+}
+
+/* Solves U c = V (n x n, row-major) by Gaussian elimination with partial pivoting.
+ * Returns 0 when a pivot falls below 1e-14 of the largest |U| entry. */
+int orc_small_solve(int n, double *U, double *V, double *c)
+{
+    double umax = 0;
+    for (int i = 0; i < n * n; i++) umax = fmax(umax, fabs(U[i]));
+    for (int col = 0; col < n; col++) {
+        int piv = col;
+        for (int r = col + 1; r < n; r++)
+            if (fabs(U[r * n + col]) > fabs(U[piv * n + col])) piv = r;
+        if (!(fabs(U[piv * n + col]) > 1e-14 * umax)) return 0;
+        if (piv != col) {
+            for (int j = 0; j < n; j++) {
+                const double t = U[col * n + j]; U[col * n + j] = U[piv * n + j]; U[piv * n + j] = t;
+            }
+            const double t = V[col]; V[col] = V[piv]; V[piv] = t;
+        }
+        for (int r = col + 1; r < n; r++) {
+            const double fct = U[r * n + col] / U[col * n + col];
+            for (int j = col; j < n; j++) U[r * n + j] -= fct * U[col * n + j];
+            V[r] -= fct * V[col];
+        }
+    }
+    for (int r = n - 1; r >= 0; r--) {
+        double s = V[r];
+        for (int j = r + 1; j < n; j++) s -= U[r * n + j] * c[j];
+        c[r] = s / U[r * n + r];
+    }
+    return 1;
+}
+
+/* One iteration on one replica row W (7M; W-, W+ updated, densities written by props).
+ * DEV = (chi(phiB - phiA) - 2W-, chi(phiA + phiB - 1)) as in solve_field (scft.cu:595-639),
+ * errs = sqrt(sum DEV^2 / 2M). Until errs < start: the reference's simple mixing
+ * W += 0.05 DEV (scft.cu:690-693). From then on, Anderson over the last H residuals:
+ * with d_k the newest residual and d_{k-m} the older ones,
+ *   U_mn = <d_k - d_{k-m}, d_k - d_{k-n}>,  V_m = <d_k - d_{k-m}, d_k>,  U c = V,
+ *   W <- sum_j a_j (W_j + lambda d_j),  a_k = 1 - sum c_m,  a_{k-m} = c_m.
+ * A singular U drops to the newest pair alone (W_k + lambda d_k). Returns errs. */
+double orc_anderson_iter(orc_grid *g, double *W, orc_anderson *a, double chi, double f, int N,
+                         double alpha, double phic, int ens, double lambda, double start)
+{
+    const int M = g->M, M2 = 2 * M, H = a->H;
+    double lnQ, phis[6];
+    orc_props(g, W, &lnQ, phis, N, alpha, phic, f, ens, NULL, NULL);
+    const double *phiA = W + 2 * M, *phiB = W + 3 * M;
+    double *d = (double *)malloc(sizeof(double) * M2);
+    double S = 0;
+    for (int r = 0; r < M; r++) {
+        d[r] = chi * (phiB[r] - phiA[r]) - 2.0 * W[r];
+        d[r + M] = chi * (phiB[r] + phiA[r] - 1.0);
+    }
+    for (int r = 0; r < M2; r++) S += d[r] * d[r];
+    const double errs = sqrt(S / M2);
+    if (!a->started && errs >= start) {
+        for (int r = 0; r < M2; r++) W[r] += 0.05 * d[r];
+        free(d);
+        return errs;
+    }
+    a->started = 1;
+    const int s = a->head, L = H + 1;
+    memcpy(a->Wr + (size_t)s * M2, W, sizeof(double) * M2);
+    memcpy(a->Dr + (size_t)s * M2, d, sizeof(double) * M2);
+    if (a->count < L) a->count++;
+    for (int m = 0; m < a->count; m++) {
+        const int j = (s - m + L) % L;
+        const double *Dj = a->Dr + (size_t)j * M2;
+        double gsum = 0;
+        for (int r = 0; r < M2; r++) gsum += d[r] * Dj[r];
+        a->G[s * L + j] = a->G[j * L + s] = gsum;
+    }
+    int n = a->count - 1;
+    double coef[64];
+    if (n > 0) {
+        double U[64 * 64], V[64], c[64];
+        for (int m = 1; m <= n; m++) {
+            const int jm = (s - m + L) % L;
+            V[m - 1] = a->G[s * L + s] - a->G[jm * L + s];
+            for (int q = 1; q <= n; q++) {
+                const int jq = (s - q + L) % L;
+                U[(m - 1) * n + (q - 1)] = a->G[s * L + s] - a->G[s * L + jq] - a->G[jm * L + s] + a->G[jm * L + jq];
+            }
+        }
+        if (!orc_small_solve(n, U, V, c)) n = 0;
+        for (int m = 0; m < n; m++) coef[m] = c[m];
+    }
+    double a0 = 1.0;
+    for (int m = 0; m < n; m++) a0 -= coef[m];
+    for (int r = 0; r < M2; r++) {
+        double v = a0 * (a->Wr[(size_t)s * M2 + r] + lambda * a->Dr[(size_t)s * M2 + r]);
+        for (int m = 1; m <= n; m++) {
+            const int jm = (s - m + L) % L;
+            v += coef[m - 1] * (a->Wr[(size_t)jm * M2 + r] + lambda * a->Dr[(size_t)jm * M2 + r]);
+        }
+        W[r] = v;
+    }
+    a->head = (s + 1) % L;
+    free(d);
+    return errs;
+}

@@ -121,6 +121,20 @@ void orc_grid_set_phiha(orc_grid *g, double phihA);
 void orc_dlnq_dbox(orc_grid *g, double *w, int Ns, double alpha, double phic, double f, int ens,
                    double *lnQ, double dlnQ[3]);
 
+/* F3: opt-in Anderson mixing for dostring = 0 (the reference keeps only its arrays, DIM /
+ * DDEV / WIN, scft.cu:30-31, 1029-1031). State of one replica with H histories. */
+typedef struct orc_anderson orc_anderson;
+orc_anderson *orc_anderson_create(int M, int H);
+void orc_anderson_destroy(orc_anderson *a);
+/* Gaussian elimination with partial pivoting; 0 when singular (pivot < 1e-14 max|U|) */
+int orc_small_solve(int n, double *U, double *V, double *c);
+/* One iteration on one replica row W (7M): props, DEV, errs; the reference's simple mixing
+ * (lambda 0.05) while errs >= start, Anderson with mixing fraction lambda after. Returns
+ * errs. Pinned by: H = 0 equals simple mixing with lambda; the fixed point equals simple
+ * mixing's; a linear residual map is solved exactly after enough histories. */
+double orc_anderson_iter(orc_grid *g, double *W, orc_anderson *a, double chi, double f, int N,
+                         double alpha, double phic, int ens, double lambda, double start);
+
 #ifdef __cplusplus
 }
 #endif

@@ -106,6 +106,9 @@ Run set_parameters(const Params& P)
     geti("device", p.device);
     geti("fe_fp64", p.fe_fp64);
     get("phihA", p.phihA);
+    geti("anderson", p.anderson);
+    get("anderson_lambda", p.anderson_lambda);
+    get("anderson_start", p.anderson_start);
     geti("relaxbox", r.relaxbox);
     get("stresstol", r.stresstol);
     if (has(P, "precision")) p.precision = P.at("precision").front() == 32 ? SCFT_FP32 : SCFT_FP64;

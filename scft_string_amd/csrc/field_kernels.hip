@@ -356,6 +356,63 @@ void launch_fe_sums(const int* d_slot2rep, int nb, long long M, double chi, cons
     hipLaunchKernelGGL(k_fe_sums, dim3(nblk, nb), dim3(FK_NT), 0, st, d_slot2rep, M, chi, W, phi, part);
 }
 
+// ---------------------------------------------------------------- Anderson mixing
+// part[blk][q] = block sums of D . ring[rows[q]] over M2 entries, q < n
+__global__ void __launch_bounds__(FK_NT) k_and_dots(const double* __restrict__ D, const double* __restrict__ ring,
+                                                    const int* __restrict__ rows, int n, long long M2,
+                                                    double* __restrict__ part)
+{
+    __shared__ double sh[FK_NT];
+    for (int q = 0; q < n; ++q) {
+        const double* Dj = ring + (size_t)rows[q] * (size_t)M2;
+        double s = 0;
+        GRID_STRIDE(r, M2) { s += D[r] * Dj[r]; }
+        s = fk_block_sum(s, sh);
+        if (threadIdx.x == 0) part[(size_t)blockIdx.x * n + q] = s;
+    }
+}
+
+// W = sum_q coef[q] (Wr[rows[q]] + lam Dr[rows[q]]), in the order q = 0, 1, ...
+__global__ void __launch_bounds__(FK_NT) k_and_combine(double* __restrict__ W, const double* __restrict__ Wr,
+                                                       const double* __restrict__ Dr, const int* __restrict__ rows,
+                                                       const double* __restrict__ coef, int n, double lam,
+                                                       long long M2)
+{
+    GRID_STRIDE(r, M2)
+    {
+        double v = 0;
+        for (int q = 0; q < n; ++q) {
+            const size_t o = (size_t)rows[q] * (size_t)M2 + r;
+            v += coef[q] * (Wr[o] + lam * Dr[o]);
+        }
+        W[r] = v;
+    }
+}
+
+// W += lam D over M2 entries
+__global__ void __launch_bounds__(FK_NT) k_axpy(double* __restrict__ W, const double* __restrict__ D, double lam,
+                                                long long M2)
+{
+    GRID_STRIDE(r, M2) { W[r] += lam * D[r]; }
+}
+
+void launch_and_dots(const double* D, const double* ring, const int* rows, int n, long long M2, double* part,
+                     int nblk, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_and_dots, dim3(nblk), dim3(FK_NT), 0, st, D, ring, rows, n, M2, part);
+}
+
+void launch_and_combine(double* W, const double* Wr, const double* Dr, const int* rows, const double* coef, int n,
+                        double lam, long long M2, int nblk, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_and_combine, dim3(nblk), dim3(FK_NT), 0, st, W, Wr, Dr, rows, coef, n, lam, M2);
+}
+
+void launch_axpy(double* W, const double* D, double lam, long long M2, int nblk, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_axpy, dim3(nblk), dim3(FK_NT), 0, st, W, D, lam, M2);
+}
+
 // part[slot][nblk] = block sums of phiA phiB (the chi derivative of FEs1, scft.cu:823-828)
 __global__ void __launch_bounds__(FK_NT) k_phiab_sums(const int* __restrict__ slot2rep, long long M,
                                                       const double* __restrict__ phi, double* __restrict__ part)

@@ -50,6 +50,14 @@ void launch_redistribute(int R, long long M, const double* d_x, const double* d_
 void launch_fe_sums(const int* d_slot2rep, int nb, long long M, double chi, const double* W, const double* phi,
                     double* part, int nblk, hipStream_t st);
 
+// Anderson mixing (opt-in, dostring = 0): dots of D with listed ring rows -> part[nblk][n];
+// W = sum_q coef[q] (Wr[rows[q]] + lam Dr[rows[q]]); W += lam D. All over M2 = 2M entries.
+void launch_and_dots(const double* D, const double* ring, const int* rows, int n, long long M2, double* part,
+                     int nblk, hipStream_t st);
+void launch_and_combine(double* W, const double* Wr, const double* Dr, const int* rows, const double* coef, int n,
+                        double lam, long long M2, int nblk, hipStream_t st);
+void launch_axpy(double* W, const double* D, double lam, long long M2, int nblk, hipStream_t st);
+
 // part [slot][nblk] = block sums of phiA phiB
 void launch_phiab_sums(const int* d_slot2rep, int nb, long long M, const double* phi, double* part, int nblk,
                        hipStream_t st);

@@ -46,6 +46,18 @@ struct scft_ctx {
     int* hslot = nullptr;   // [R]
     double* hsmall = nullptr;  // [4R] x, xref, lam, cproj
     int* hact = nullptr;    // [R]
+    // Anderson mixing (opt-in, dostring = 0): per replica rings of the last H+1 (W, DEV)
+    // pairs [R][H+1][2M], the Gram sums of their DEVs on the host, and small staging
+    struct Anderson {
+        int count = 0, head = 0;
+        bool started = false;
+        std::vector<double> G;  // [(H+1)^2]
+    };
+    std::vector<Anderson> and_;
+    double *dAW = nullptr, *dAD = nullptr, *dApart = nullptr, *dAcoef = nullptr;
+    int* dArows = nullptr;
+    double* hAbuf = nullptr;  // [(H+1) (nblk + 1)] partials and coefficients
+    int* hArows = nullptr;    // [H+1]
 };
 
 namespace {
@@ -130,6 +142,46 @@ scft_status edist2(scft_ctx* c, std::vector<double>& del2)
     return finalize_to_host(c, c->R - 1, 1, del2);
 }
 
+// Anderson histories describe the fields the iteration produced; any outside change of the
+// fields or the box starts them again.
+void anderson_reset(scft_ctx* c)
+{
+    for (auto& a : c->and_) {
+        a.count = a.head = 0;
+        a.started = false;
+    }
+}
+
+// Solves U c = V (n x n, row-major) by Gaussian elimination with partial pivoting; false
+// when a pivot falls below 1e-14 of the largest |U| entry.
+bool small_solve(int n, std::vector<double>& U, std::vector<double>& V, std::vector<double>& x)
+{
+    double umax = 0;
+    for (double u : U) umax = std::max(umax, std::fabs(u));
+    for (int col = 0; col < n; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < n; ++r)
+            if (std::fabs(U[r * n + col]) > std::fabs(U[piv * n + col])) piv = r;
+        if (!(std::fabs(U[piv * n + col]) > 1e-14 * umax)) return false;
+        if (piv != col) {
+            for (int j = 0; j < n; ++j) std::swap(U[col * n + j], U[piv * n + j]);
+            std::swap(V[col], V[piv]);
+        }
+        for (int r = col + 1; r < n; ++r) {
+            const double fct = U[r * n + col] / U[col * n + col];
+            for (int j = col; j < n; ++j) U[r * n + j] -= fct * U[col * n + j];
+            V[r] -= fct * V[col];
+        }
+    }
+    x.assign(n, 0.0);
+    for (int r = n - 1; r >= 0; --r) {
+        double s = V[r];
+        for (int j = r + 1; j < n; ++j) s -= U[r * n + j] * x[j];
+        x[r] = s / U[r * n + r];
+    }
+    return true;
+}
+
 // scft.cu:612-621 / 725-736: normalised arc-length positions from W- distances
 void arc_positions(int R, const std::vector<double>& del2, std::vector<double>& x)
 {
@@ -145,6 +197,124 @@ void arc_positions(int R, const std::vector<double>& del2, std::vector<double>& 
     x[0] = 0;
     x[R - 1] = 1;
     for (int i = 1; i < R - 1; ++i) x[i] = x[i - 1] + Del[i - 1];
+}
+
+// One iteration of the opt-in Anderson mode (dostring = 0). Every active replica gets a
+// props and its residual DEV = (chi(phiB - phiA) - 2W-, chi(phiA + phiB - 1)) (scft.cu:595-639
+// without the spring). Then each replica takes the reference's simple mixing W += 0.05 DEV
+// (scft.cu:690-693) while errs >= start, and Anderson mixing after. With d_k the newest
+// residual and d_{k-m} the m-th older one in its ring:
+//   U_mn = <d_k - d_{k-m}, d_k - d_{k-n}>,  V_m = <d_k - d_{k-m}, d_k>,  U c = V,
+//   W <- sum_j a_j (W_j + lambda d_j),  a_k = 1 - sum c_m,  a_{k-m} = c_m,
+// falling back to the newest pair alone when U is singular.
+scft_status anderson_iterate(scft_ctx* c, double* alf, scft_iter_report* rep)
+{
+    const int R = c->R, L = c->p.anderson + 1;
+    const long long M = c->M, M2 = 2 * M;
+    const double lam = c->p.anderson_lambda > 0 ? c->p.anderson_lambda : 0.2;
+    // Started too far from a solution, Anderson can jump to another saddle point (a lamella
+    // at errs 0.1 went to the disordered state), so simple mixing runs until errs < 1e-2.
+    const double start = c->p.anderson_start > 0 ? c->p.anderson_start : 1e-2;
+    const std::vector<int> doiis = fixed_doiis(c);
+    std::vector<int> active;
+    for (int ii = 0; ii < R; ++ii)
+        if (doiis[ii]) active.push_back(ii);
+    const int nb = (int)active.size();
+    std::vector<double> errs(R, 0.0);
+    std::vector<int> steps(R, 0);
+    if (nb > 0) {
+        SCFTCHK(run_props(c, c->dW, 2 * M, c->dphi, 4 * M, active, nullptr, nullptr));
+        scft::launch_dev_full(c->dslot, nb, R, M, c->p.chi, 0, 0.5, c->dphi, c->dW, c->dDEV, c->dpart, c->nblk,
+                              c->st);
+        HIPCHK(c, hipGetLastError());
+        std::vector<double> devs;  // [slot][2]: sum DEV^2 (2M), sum DEV-^2 (M)
+        SCFTCHK(finalize_to_host(c, nb, 2, devs));
+        for (int j = 0; j < nb; ++j) {
+            errs[active[j]] = std::sqrt(devs[2 * j] / (double)M2);
+            steps[active[j]] = 1;
+        }
+    }
+    for (const int ii : active) {
+        double* W = c->dW + (size_t)ii * M2;
+        const double* D = c->dDEV + (size_t)ii * M2;
+        auto& A = c->and_[ii];
+        if (!A.started && errs[ii] >= start) {
+            scft::launch_axpy(W, D, 0.05, M2, c->nblk, c->st);
+            HIPCHK(c, hipGetLastError());
+            continue;
+        }
+        A.started = true;
+        const int s = A.head;
+        double* Wr = c->dAW + (size_t)ii * L * M2;
+        double* Dr = c->dAD + (size_t)ii * L * M2;
+        HIPCHK(c, hipMemcpyAsync(Wr + (size_t)s * M2, W, sizeof(double) * M2, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(Dr + (size_t)s * M2, D, sizeof(double) * M2, hipMemcpyDeviceToDevice, c->st));
+        if (A.count < L) ++A.count;
+        // Gram sums of the newest residual with every stored one, newest first
+        for (int m = 0; m < A.count; ++m) c->hArows[m] = (s - m + L) % L;
+        HIPCHK(c, hipMemcpyAsync(c->dArows, c->hArows, sizeof(int) * A.count, hipMemcpyHostToDevice, c->st));
+        scft::launch_and_dots(Dr + (size_t)s * M2, Dr, c->dArows, A.count, M2, c->dApart, c->nblk, c->st);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemcpyAsync(c->hAbuf, c->dApart, sizeof(double) * A.count * c->nblk, hipMemcpyDeviceToHost,
+                                 c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        for (int m = 0; m < A.count; ++m) {
+            double g = 0;
+            for (int b = 0; b < c->nblk; ++b) g += c->hAbuf[(size_t)b * A.count + m];
+            const int j = c->hArows[m];
+            A.G[(size_t)s * L + j] = A.G[(size_t)j * L + s] = g;
+        }
+        int n = A.count - 1;
+        std::vector<double> cm;
+        if (n > 0) {
+            std::vector<double> U((size_t)n * n), V(n);
+            const double gss = A.G[(size_t)s * L + s];
+            for (int m = 1; m <= n; ++m) {
+                const int jm = c->hArows[m];
+                V[m - 1] = gss - A.G[(size_t)jm * L + s];
+                for (int q = 1; q <= n; ++q) {
+                    const int jq = c->hArows[q];
+                    U[(size_t)(m - 1) * n + (q - 1)] =
+                        gss - A.G[(size_t)s * L + jq] - A.G[(size_t)jm * L + s] + A.G[(size_t)jm * L + jq];
+                }
+            }
+            if (!small_solve(n, U, V, cm)) n = 0;
+        }
+        double a0 = 1.0;
+        for (int m = 0; m < n; ++m) a0 -= cm[m];
+        c->hAbuf[0] = a0;
+        for (int m = 0; m < n; ++m) c->hAbuf[1 + m] = cm[m];
+        HIPCHK(c, hipMemcpyAsync(c->dAcoef, c->hAbuf, sizeof(double) * (n + 1), hipMemcpyHostToDevice, c->st));
+        scft::launch_and_combine(W, Wr, Dr, c->dArows, c->dAcoef, n + 1, lam, M2, c->nblk, c->st);
+        HIPCHK(c, hipGetLastError());
+        A.head = (s + 1) % L;
+    }
+    std::vector<double> del2, x;
+    SCFTCHK(edist2(c, del2));
+    arc_positions(R, del2, x);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (alf)
+        for (int ii = 0; ii < R; ++ii) alf[ii] = x[ii];
+    if (rep) {
+        double stepav = 0;
+        for (int ii = 0; ii < R; ++ii) stepav += steps[ii];
+        auto put = [R](double* dst, const std::vector<double>& v) {
+            if (dst) std::memcpy(dst, v.data(), sizeof(double) * R);
+        };
+        const std::vector<double> zero(R, 0.0);
+        put(rep->errs, errs);
+        put(rep->errs3, errs);
+        put(rep->x, x);
+        put(rep->xref, zero);
+        put(rep->dotWs, zero);
+        put(rep->dotWs2, zero);
+        put(rep->normDER, zero);
+        put(rep->normDEV, zero);
+        if (rep->steps) std::memcpy(rep->steps, steps.data(), sizeof(int) * R);
+        rep->fitup = 0;
+        rep->stepav = stepav / R;
+    }
+    return SCFT_OK;
 }
 
 }  // namespace
@@ -189,6 +359,7 @@ scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
     // the A homopolymer reuses q1 up to Nh, which is all-A only for Nh <= n1
     if (p->phihA < 0 || (p->phihA > 0 && c->cc.Nh > c->cc.n1)) return SCFT_ERR_INVALID;
     if (p->ens == 1 && p->phidb + p->phihA > 1.0) return SCFT_ERR_INVALID;
+    if (p->anderson < 0 || p->anderson > 40 || (p->anderson > 0 && p->dostring != 0)) return SCFT_ERR_INVALID;
 
     if (hipSetDevice(p->device) != hipSuccess) return SCFT_ERR_HIP;
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return SCFT_ERR_HIP;
@@ -228,6 +399,19 @@ scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
               hipHostMalloc((void**)&c->hslot, sizeof(int) * R, hipHostMallocDefault) == hipSuccess &&
               hipHostMalloc((void**)&c->hsmall, sizeof(double) * 4 * R, hipHostMallocDefault) == hipSuccess &&
               hipHostMalloc((void**)&c->hact, sizeof(int) * R, hipHostMallocDefault) == hipSuccess;
+    if (ok && p->anderson > 0) {
+        const size_t L = (size_t)p->anderson + 1;
+        ok = hipMalloc((void**)&c->dAW, sizeof(double) * L * 2 * M * R) == hipSuccess &&
+             hipMalloc((void**)&c->dAD, sizeof(double) * L * 2 * M * R) == hipSuccess &&
+             hipMalloc((void**)&c->dApart, sizeof(double) * L * c->nblk) == hipSuccess &&
+             hipMalloc((void**)&c->dAcoef, sizeof(double) * L) == hipSuccess &&
+             hipMalloc((void**)&c->dArows, sizeof(int) * L) == hipSuccess &&
+             hipHostMalloc((void**)&c->hAbuf, sizeof(double) * L * (c->nblk + 1), hipHostMallocDefault) ==
+                 hipSuccess &&
+             hipHostMalloc((void**)&c->hArows, sizeof(int) * L, hipHostMallocDefault) == hipSuccess;
+        c->and_.assign(R, scft_ctx::Anderson{});
+        for (auto& a : c->and_) a.G.assign(L * L, 0.0);
+    }
     if (!ok) {
         scft_ctx_destroy(c.release());
         return SCFT_ERR_NOMEM;
@@ -250,11 +434,12 @@ scft_status scft_ctx_destroy(scft_ctx* c)
     c->pe32.reset();
     c->pe64.reset();
     c->pe64_fe.reset();
-    void* dptrs[] = {c->dW, c->dphi, c->dDEV, c->ddWda, c->dWs, c->dphis_rows, c->dpart, c->dsm, c->dlam,
-                     c->dx, c->dxref, c->dcproj, c->dphisv, c->dactive, c->dslot};
+    void* dptrs[] = {c->dW,    c->dphi,  c->dDEV,   c->ddWda,   c->dWs,    c->dphis_rows, c->dpart,
+                     c->dsm,   c->dlam,  c->dx,     c->dxref,   c->dcproj, c->dphisv,     c->dactive,
+                     c->dslot, c->dAW,   c->dAD,    c->dApart,  c->dAcoef, c->dArows};
     for (void* q : dptrs)
         if (q) (void)hipFree(q);
-    void* hptrs[] = {c->hsm, c->hslot, c->hsmall, c->hact};
+    void* hptrs[] = {c->hsm, c->hslot, c->hsmall, c->hact, c->hAbuf, c->hArows};
     for (void* q : hptrs)
         if (q) (void)hipHostFree(q);
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -304,6 +489,7 @@ scft_status scft_set_fields(scft_ctx* c, const double* W)
     if (!c || !W) return fail(c, SCFT_ERR_INVALID, "scft_set_fields: null");
     HIPCHK(c, hipMemcpyAsync(c->dW, W, sizeof(double) * 2 * c->M * c->R, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    anderson_reset(c);
     return SCFT_OK;
 }
 
@@ -335,6 +521,7 @@ scft_status scft_get_tangent(scft_ctx* c, double* dWda)
 scft_status scft_solve_w_plus(scft_ctx* c, int maxIter, double errTol, const int* which, int* k_out)
 {
     if (!c) return SCFT_ERR_INVALID;
+    anderson_reset(c);  // W+ changes outside the Anderson iteration
     const int R = c->R;
     const long long M = c->M;
     std::vector<int> act(R), k(R, 1);
@@ -378,6 +565,7 @@ scft_status scft_solve_w_plus(scft_ctx* c, int maxIter, double errTol, const int
 scft_status scft_string_iterate(scft_ctx* c, int k, double errTol, double* alf, scft_iter_report* rep)
 {
     if (!c) return SCFT_ERR_INVALID;
+    if (c->p.anderson > 0) return anderson_iterate(c, alf, rep);
     const int R = c->R;
     const long long M = c->M;
     const double lambda = 0.05, eps = 0.5;  // scft.cu:562, 570 (lambda stays 0.05: A.1)
@@ -663,6 +851,7 @@ scft_status scft_set_box(scft_ctx* c, const double D[3])
     if (e == hipSuccess && c->pe64_fe) e = c->pe64_fe->set_box(D);
     if (e != hipSuccess) return fail(c, SCFT_ERR_HIP, std::string("scft_set_box: ") + hipGetErrorString(e));
     for (int i = 0; i < 3; ++i) c->p.D[i] = D[i];
+    anderson_reset(c);
     return SCFT_OK;
 }
 
