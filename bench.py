@@ -177,6 +177,31 @@ def pmc_probe():
     print("pmc probe done", flush=True)
 
 
+def batched_fp64_rate(W0, warmup, steps):
+    """The same string with an FP64 propagator (every replica batched on the GPU): string
+    iterations per second over `steps` iterations after `warmup`, and the contour step."""
+    from scft_string_amd import Context
+
+    c = CFG
+    R = c["nrep"]
+    ctx = Context(c["m"], c["D"], c["N"], c["f"], c["alpha"], c["ens"], c["phidb"], c["chi"], nrep=R,
+                  precision="fp64", fix=c["fix"], dostring=1)
+    ctx.set_fields(W0)
+    for k in range(warmup):
+        ctx.string_iterate(k)
+    ctx.time_props(1, reps=1)
+    t0 = time.perf_counter()
+    for k in range(warmup, warmup + steps):
+        ctx.string_iterate(k)
+    dt = (time.perf_counter() - t0) / steps
+    n_act = R - sum(c["fix"])
+    ms_step = ctx.time_props(n_act, reps=2)
+    ctx.close()
+    bstep = algorithmic_bytes_per_step(c["m"], 8) * n_act
+    return {"iters_per_s": 1.0 / dt, "ms_per_contour_step": ms_step,
+            "frac": bstep / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, "sample": f"{steps} iterations after {warmup}"}
+
+
 def adjoint_report():
     """Box derivative d lnQ/dD (adjoint of props) at 64^3, N = 100, chiN = 20, f = 0.5, FP32,
     on a 1-replica and an 8-replica string (lamellar -> cylinder interpolants, seed 2024),
@@ -295,6 +320,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the batched FP64 rate of the same string")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--pmc-probe", action="store_true")
     ap.add_argument("--adjoint", action="store_true",
@@ -377,6 +403,8 @@ def main():
                                f"{n_act} replicas", "ms_per_contour_step": ms_step,
                      "algorithmic_bytes_per_contour_step": bstep},
     }
+    if a.precision == "fp32" and not a.no_fp64:
+        out["batched_fp64"] = batched_fp64_rate(W0, a.warmup, 2)
     if not a.no_plain:
         pl = plain_fp64_rate(Wend, c, props / a.steps)
         pl["speedup_of_value"] = iters_per_s / pl["iters_per_s"]

@@ -70,6 +70,10 @@ typedef struct scft_params {
     double anderson_start;   /* simple mixing (lambda 0.05) until errs < this (<= 0: 1e-2;
                                 started much earlier, Anderson can land on another saddle
                                 point, e.g. the disordered one) */
+    int autotune;            /* 1: the first props times every kernel family (two-stage, LDS
+                                Stockham, line kernels) x replica-group streams (1-3) x graph
+                                replay on that batch and keeps the fastest. FP32 results then
+                                depend on the choice (the families round differently). */
 } scft_params;
 
 typedef struct scft_ctx scft_ctx;
@@ -85,6 +89,9 @@ scft_status scft_sync(scft_ctx *ctx);
  * compiled for the grid) or "pencil" (one line pass per axis, any 2-3-5 grid). The
  * reference has no counterpart: cuFFT picks its own plan (density.h:41-75). */
 const char *scft_fft_path(const scft_ctx *ctx);
+/* The propagator configuration in words ("two-stage, 2 replica-group streams, direct
+ * launches"), followed after an autotune by every candidate's contour-loop time in ms. */
+const char *scft_engine_info(scft_ctx *ctx);
 
 /* Replaces density::props (density.h:344-459), batched over `nrep` independent rows:
  * W [nrep][2M] in; phi [nrep][4M], lnQ [nrep], phis [nrep][5] out (phi, phis may be NULL).

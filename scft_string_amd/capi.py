@@ -20,9 +20,9 @@ EXPORTED = [
     "scft_props_batch_device", "scft_set_fields", "scft_get_fields", "scft_get_densities",
     "scft_get_tangent", "scft_solve_w_plus", "scft_string_iterate", "scft_free_energy",
     "scft_debug_qq_invariant", "scft_debug_time_props", "scft_fft_path", "scft_dlnq_dbox",
-    "scft_box_stress", "scft_set_box", "scft_relax_box", "scft_iterate_concurrent",
+    "scft_box_stress", "scft_set_box", "scft_relax_box", "scft_iterate_concurrent", "scft_engine_info",
 ]
-_STRING_RESULT = ("scft_last_error", "scft_fft_path")
+_STRING_RESULT = ("scft_last_error", "scft_fft_path", "scft_engine_info")
 
 
 class ScftError(RuntimeError):
@@ -38,7 +38,7 @@ class Params(C.Structure):
         ("nrep", C.c_int), ("precision", C.c_int), ("fix", C.c_int * 2), ("dostring", C.c_int),
         ("justFE", C.c_int), ("springsteps", C.c_int), ("device", C.c_int), ("fe_fp64", C.c_int),
         ("phihA", C.c_double), ("anderson", C.c_int), ("anderson_lambda", C.c_double),
-        ("anderson_start", C.c_double),
+        ("anderson_start", C.c_double), ("autotune", C.c_int),
     ]
 
 
@@ -79,6 +79,8 @@ def lib():
         L.scft_debug_time_props.argtypes = [vp, C.c_int, C.c_int, dp]
         L.scft_fft_path.argtypes = [vp]
         L.scft_fft_path.restype = C.c_char_p
+        L.scft_engine_info.argtypes = [vp]
+        L.scft_engine_info.restype = C.c_char_p
         L.scft_dlnq_dbox.argtypes = [vp, dp, C.c_int, dp, dp]
         L.scft_box_stress.argtypes = [vp, dp, dp]
         L.scft_set_box.argtypes = [vp, dp]
@@ -102,7 +104,7 @@ class Context:
 
     def __init__(self, m, D, N, f, alpha, ens, phidb, chi, nrep=1, precision="fp64", fix=(0, 0),
                  dostring=1, justFE=0, springsteps=0, device=0, fe_fp64=1, phihA=0.0, anderson=0,
-                 anderson_lambda=0.0, anderson_start=0.0):
+                 anderson_lambda=0.0, anderson_start=0.0, autotune=0):
         self.m = tuple(int(v) for v in m)
         self.D = tuple(float(v) for v in D)
         self.M = int(np.prod(self.m))
@@ -114,7 +116,7 @@ class Context:
                    precision=prec, fix=(C.c_int * 2)(*fix), dostring=int(dostring), justFE=int(justFE),
                    springsteps=int(springsteps), device=int(device), fe_fp64=int(fe_fp64),
                    phihA=float(phihA), anderson=int(anderson), anderson_lambda=float(anderson_lambda),
-                   anderson_start=float(anderson_start))
+                   anderson_start=float(anderson_start), autotune=int(autotune))
         self.params = p
         h = C.c_void_p()
         rc = lib().scft_ctx_create(C.byref(p), C.byref(h))
@@ -147,6 +149,11 @@ class Context:
     def fft_path(self):
         """"two-stage", "stockham" or "pencil": the propagator's FFT kernels for this grid."""
         return lib().scft_fft_path(self._h).decode()
+
+    @property
+    def engine_info(self):
+        """The propagator configuration, with the autotuner's timings when it ran."""
+        return lib().scft_engine_info(self._h).decode()
 
     # ---------------------------------------------------------------- props (density.h:344)
     def props_batch(self, W):

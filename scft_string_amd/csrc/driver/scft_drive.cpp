@@ -107,6 +107,7 @@ Run set_parameters(const Params& P)
     geti("fe_fp64", p.fe_fp64);
     get("phihA", p.phihA);
     geti("anderson", p.anderson);
+    geti("autotune", p.autotune);
     get("anderson_lambda", p.anderson_lambda);
     get("anderson_start", p.anderson_start);
     geti("relaxbox", r.relaxbox);

@@ -323,13 +323,93 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
     use_graph_ = gr && gr[0] == '1';
     const char* ns = getenv("SCFT_STREAMS");
     nstreams_ = std::max(1, std::min(kMaxStreams, ns ? atoi(ns) : kDefaultStreams));
-    // extra streams at the default priority, plus fork/join events (no timing)
+    // extra streams at the default priority, plus fork/join events (no timing); all of them,
+    // so that the autotuner can pick any replica-group count
     if (hipEventCreateWithFlags(&fork_, hipEventDisableTiming) != hipSuccess) return "event creation failed";
-    for (int i = 0; i + 1 < nstreams_; ++i)
+    for (int i = 0; i + 1 < kMaxStreams; ++i)
         if (hipStreamCreateWithFlags(&sx_[i], hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&join_[i], hipEventDisableTiming) != hipSuccess)
             return "stream / event creation failed";
+    // kernel families this grid can run, for the autotuner
+    fams_.clear();
+    if (ks_ && ks_->step2) fams_.push_back({"two-stage", ks_->setup2, ks_->begin2, ks_->step2});
+    if (ks_ && ks_->step != ks_->step2) fams_.push_back({"stockham", ks_->setup, ks_->begin, ks_->step});
+    if (pencil_supported(m[0], m[1], m[2]))
+        fams_.push_back({"pencil", &pencil_setup<T>, &pencil_begin<T>, &pencil_step<T>});
     return "";
+}
+
+// Times every configuration of the contour loop on the batch at hand (same fields, nb, cc):
+// kernel family x replica-group streams (1 .. 3) x graph replay, best of two runs each with
+// HIP events on st, and keeps the fastest. The contour loop recomputes everything it
+// writes, so the trial runs leave no trace.
+template <typename T>
+hipError_t PropEngine<T>::tune(const PropArgs<T>& a, int nb, const ChainConst& cc, hipStream_t st)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    struct Choice {
+        int fam, streams;
+        bool graph;
+    };
+    Choice best{-1, 1, false};
+    float best_ms = 0;
+    tune_log_.clear();
+    for (int f = 0; f < (int)fams_.size() && e == hipSuccess; ++f) {
+        if ((e = fams_[f].setup()) != hipSuccess) break;
+        for (int ns = 1; ns <= std::min(3, kMaxStreams) && e == hipSuccess; ++ns) {
+            if (ns > 1 && ns > nb) break;
+            for (int gph = 0; gph < 2 && e == hipSuccess; ++gph) {
+                begin_ = fams_[f].begin;
+                step_ = fams_[f].step;
+                nstreams_ = ns;
+                use_graph_ = gph == 1;
+                for (auto& g : graphs_) (void)hipGraphExecDestroy(g.exec);
+                graphs_.clear();
+                float t = -1;
+                for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+                    if ((e = hipEventRecord(e0, st)) != hipSuccess) break;
+                    if ((e = contour(a, nb, cc, st)) != hipSuccess) break;
+                    if ((e = hipEventRecord(e1, st)) != hipSuccess) break;
+                    if ((e = hipEventSynchronize(e1)) != hipSuccess) break;
+                    float ms = 0;
+                    if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+                    t = t < 0 ? ms : std::min(t, ms);
+                }
+                if (e != hipSuccess) break;
+                char line[128];
+                snprintf(line, sizeof line, "%s/%d/%s %.3f; ", fams_[f].name, ns, gph ? "graph" : "direct", t);
+                tune_log_ += line;
+                if (best.fam < 0 || t < best_ms) {
+                    best = {f, ns, gph == 1};
+                    best_ms = t;
+                }
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess || best.fam < 0) return e != hipSuccess ? e : hipErrorUnknown;
+    if ((e = fams_[best.fam].setup()) != hipSuccess) return e;
+    begin_ = fams_[best.fam].begin;
+    step_ = fams_[best.fam].step;
+    path_ = fams_[best.fam].name;
+    nstreams_ = best.streams;
+    use_graph_ = best.graph;
+    for (auto& g : graphs_) (void)hipGraphExecDestroy(g.exec);
+    graphs_.clear();
+    tuned_ = true;
+    return hipSuccess;
+}
+
+template <typename T>
+std::string PropEngine<T>::config() const
+{
+    char b[160];
+    snprintf(b, sizeof b, "%s, %d replica-group stream%s, %s%s", path_, nstreams_, nstreams_ > 1 ? "s" : "",
+             use_graph_ ? "graph replay" : "direct launches", tuned_ ? " (autotuned)" : "");
+    return b;
 }
 
 // Per-axis kinetic tables for a_i = (2 pi k_i / D_i)^2 ds / 12 with k_i the signed frequency
@@ -365,6 +445,11 @@ hipError_t PropEngine<T>::run(const double* dW, long long wstride, double* dphi,
     hipLaunchKernelGGL(k_wmean, dim3(nb), dim3(RED_NT), 0, st, part_, nblk_, M_, cAB_);
     hipLaunchKernelGGL((k_factors<T>), rg, dim3(RED_NT), 0, st, dW, wstride, d_slot2rep, M_, cc.ds, cAB_, e4A_,
                        e4B_);
+    if (tune_pending_) {
+        tune_pending_ = false;
+        hipError_t et = tune(a, nb, cc, st);
+        if (et != hipSuccess) return et;
+    }
     if (timing_) (void)hipEventRecord(ev_[0], st);
     hipError_t e = contour(a, nb, cc, st);
     if (e != hipSuccess) return e;

@@ -64,6 +64,13 @@ public:
     int nbmax() const { return nbmax_; }
     size_t bytes_allocated() const { return bytes_; }
     const char* fft_path() const { return path_; }
+    // The next run() first times every kernel family x stream count x graph replay on its
+    // batch and keeps the fastest (see tune()).
+    void request_autotune() { tune_pending_ = true; }
+    // "two-stage, 2 replica-group streams, direct launches (autotuned)"
+    std::string config() const;
+    // per-candidate timings of the last autotune ("family/streams/mode ms; ...")
+    const std::string& tune_log() const { return tune_log_; }
     // HIP events around the contour loop of the next run() calls (diagnostics)
     hipError_t enable_timing(bool on);
     // elapsed ms of the contour loop of the last run() (after the stream is synchronised)
@@ -71,6 +78,16 @@ public:
 
 private:
     PropArgs<T> args(int nb, const ChainConst& cc) const;
+    hipError_t tune(const PropArgs<T>& a, int nb, const ChainConst& cc, hipStream_t st);
+    struct Family {
+        const char* name;
+        hipError_t (*setup)();
+        void (*begin)(const PropArgs<T>&, hipStream_t);
+        void (*step)(const PropArgs<T>&, int s, hipStream_t);
+    };
+    std::vector<Family> fams_;
+    bool tune_pending_ = false, tuned_ = false;
+    std::string tune_log_;
     hipError_t contour(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st);
     hipError_t contour_launch(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st);
     struct GraphEntry {

@@ -28,6 +28,7 @@ struct scft_ctx {
     ChainConst cc{};
     hipStream_t st = nullptr;
     std::string err;
+    std::string info;  // scft_engine_info
     std::unique_ptr<PropEngine<float>> pe32;
     std::unique_ptr<PropEngine<double>> pe64;
     // FP32 contexts: FP64 engine for the props that FreeE takes F from (created on first use)
@@ -330,6 +331,16 @@ const char* scft_fft_path(const scft_ctx* ctx)
     return ctx->pe64 ? ctx->pe64->fft_path() : "";
 }
 
+const char* scft_engine_info(scft_ctx* ctx)
+{
+    if (!ctx) return "";
+    if (ctx->pe32) ctx->info = ctx->pe32->config() + (ctx->pe32->tune_log().empty() ? "" : "; tuned: ") +
+                               ctx->pe32->tune_log();
+    else if (ctx->pe64) ctx->info = ctx->pe64->config() + (ctx->pe64->tune_log().empty() ? "" : "; tuned: ") +
+                                    ctx->pe64->tune_log();
+    return ctx->info.c_str();
+}
+
 scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
 {
     if (!p || !out) return SCFT_ERR_INVALID;
@@ -370,6 +381,10 @@ scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
     } else {
         c->pe64 = std::make_unique<PropEngine<double>>();
         msg = c->pe64->init(p->m, p->D, p->N, c->R);
+    }
+    if (msg.empty() && p->autotune) {
+        if (c->pe32) c->pe32->request_autotune();
+        else c->pe64->request_autotune();
     }
     if (!msg.empty()) {
         fprintf(stderr, "scft_ctx_create: %s\n", msg.c_str());
