@@ -22,7 +22,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0
+from scft_string_amd.timing import HBM_PEAK_GBS, algorithmic_bytes_per_step  # noqa: E402
 
 CFG = dict(m=(64, 64, 64), D=(6.0, 6.0, 6.0), N=128, chi=18.0, f=0.45, phidb=0.9, ens=1, alpha=0.1,
            nrep=16, fix=(1, 1))
@@ -45,14 +45,6 @@ def initial_fields(m, D, R, seed=2024):
         if 0 < i < R - 1:
             W[i, 0] += 0.01 * rng.standard_normal(M)
     return W
-
-
-def algorithmic_bytes_per_step(m, p):
-    """12 (M p + Mk 2p): 2 chains x 6 real 3-D FFTs, each reading its input and writing its
-    output once, pointwise factors fused (per contour step per replica)."""
-    M = m[0] * m[1] * m[2]
-    Mk = m[0] * m[1] * (m[2] // 2 + 1)
-    return 12 * (M * p + Mk * 2 * p)
 
 
 def cpu_baseline(m, D, N, R_active, props_per_iter, budget_s=20.0):

@@ -154,8 +154,15 @@ template <typename T>
 static hipError_t pencil_setup()
 {
     const int lds = (int)pencil_lds_max<T>();  // shared by every context: never one grid's need
-    const void* fns[] = {(const void*)k_pz<T>, (const void*)k_py<T, false>, (const void*)k_py<T, true>,
-                         (const void*)k_px<T>};
+    std::vector<const void*> fns = {(const void*)k_pz<T, 0>, (const void*)k_py<T, false, 0>,
+                                    (const void*)k_py<T, true, 0>, (const void*)k_px<T, 0>};
+#define PC_FNS(L)                                                                                                 \
+    fns.push_back((const void*)k_pz<T, L>);                                                                       \
+    fns.push_back((const void*)k_py<T, false, L>);                                                                \
+    fns.push_back((const void*)k_py<T, true, L>);                                                                 \
+    fns.push_back((const void*)k_px<T, L>);
+    PC_CT_LENGTHS(PC_FNS)
+#undef PC_FNS
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         if (e != hipSuccess) return e;
@@ -163,12 +170,20 @@ static hipError_t pencil_setup()
     return hipSuccess;
 }
 
+// Each pass launches the kernel compiled for its axis length when there is one.
 template <typename T>
 static void pencil_pz(const PropArgs<T>& a, int s, int mode, const PencilTiles& p, size_t lds, hipStream_t st)
 {
     const long long lines = (long long)a.m0 * a.m1;
-    hipLaunchKernelGGL((k_pz<T>), dim3((unsigned)((lines + p.nlz - 1) / p.nlz), a.nb), dim3(PC_NT), lds, st, a, s,
-                       mode, p.nlz);
+    const dim3 g((unsigned)((lines + p.nlz - 1) / p.nlz), a.nb);
+    if (a.pencil_ct) switch (a.m2) {
+#define PC_CASE(L) \
+    case L: hipLaunchKernelGGL((k_pz<T, L>), g, dim3(PC_NT), lds, st, a, s, mode, p.nlz); return;
+            PC_CT_LENGTHS(PC_CASE)
+#undef PC_CASE
+            default: break;
+        }
+    hipLaunchKernelGGL((k_pz<T, 0>), g, dim3(PC_NT), lds, st, a, s, mode, p.nlz);
 }
 
 // y pass over nfields fields (the first nh are Ch)
@@ -176,21 +191,37 @@ template <typename T, bool INV>
 static void pencil_py(const PropArgs<T>& a, int nh, int nfields, const PencilTiles& p, size_t lds, hipStream_t st)
 {
     const int ntz = (a.m2 + p.ty - 1) / p.ty;
-    hipLaunchKernelGGL((k_py<T, INV>), dim3(a.m0 * ntz, nfields), dim3(PC_NT), lds, st, a, nh, p.ty);
+    const dim3 g(a.m0 * ntz, nfields);
+    if (a.pencil_ct) switch (a.m1) {
+#define PC_CASE(L) \
+    case L: hipLaunchKernelGGL((k_py<T, INV, L>), g, dim3(PC_NT), lds, st, a, nh, p.ty); return;
+            PC_CT_LENGTHS(PC_CASE)
+#undef PC_CASE
+            default: break;
+        }
+    hipLaunchKernelGGL((k_py<T, INV, 0>), g, dim3(PC_NT), lds, st, a, nh, p.ty);
 }
 
 template <typename T>
 static void pencil_px(const PropArgs<T>& a, int nh, int nfields, const PencilTiles& p, size_t lds, hipStream_t st)
 {
     const int ntz = (a.m2 + p.tx - 1) / p.tx;
-    hipLaunchKernelGGL((k_px<T>), dim3(a.m1 * ntz, nfields), dim3(PC_NT), lds, st, a, nh, p.tx);
+    const dim3 g(a.m1 * ntz, nfields);
+    if (a.pencil_ct) switch (a.m0) {
+#define PC_CASE(L) \
+    case L: hipLaunchKernelGGL((k_px<T, L>), g, dim3(PC_NT), lds, st, a, nh, p.tx); return;
+            PC_CT_LENGTHS(PC_CASE)
+#undef PC_CASE
+            default: break;
+        }
+    hipLaunchKernelGGL((k_px<T, 0>), g, dim3(PC_NT), lds, st, a, nh, p.tx);
 }
 
 template <typename T>
 static void pencil_begin(const PropArgs<T>& a, hipStream_t st)
 {
-    const PencilTiles p = pencil_tiles(a.m0, a.m1, a.m2);
-    const size_t lds = pencil_lds_bytes<T>(a.m0, a.m1, a.m2);
+    const PencilTiles p = pencil_tiles(a.m0, a.m1, a.m2, a.pencil_ct);
+    const size_t lds = pencil_lds_bytes<T>(a.m0, a.m1, a.m2, a.pencil_ct);
     pencil_pz<T>(a, 0, PZ_BEGIN, p, lds, st);
     pencil_py<T, false>(a, a.nb, 2 * a.nb, p, lds, st);
 }
@@ -198,8 +229,8 @@ static void pencil_begin(const PropArgs<T>& a, hipStream_t st)
 template <typename T>
 static void pencil_step(const PropArgs<T>& a, int s, hipStream_t st)
 {
-    const PencilTiles p = pencil_tiles(a.m0, a.m1, a.m2);
-    const size_t lds = pencil_lds_bytes<T>(a.m0, a.m1, a.m2);
+    const PencilTiles p = pencil_tiles(a.m0, a.m1, a.m2, a.pencil_ct);
+    const size_t lds = pencil_lds_bytes<T>(a.m0, a.m1, a.m2, a.pencil_ct);
     const bool last = s == a.N - 1;
     pencil_px<T>(a, a.nb, 2 * a.nb, p, lds, st);
     pencil_py<T, true>(a, a.nb, a.nb, p, lds, st);
@@ -274,6 +305,8 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
     // Plane/column kernels where this grid has them; the line kernels otherwise, or when
     // SCFT_PENCIL=1 is set.
     const char* pen = getenv("SCFT_PENCIL");
+    const char* pen_rt = getenv("SCFT_PENCIL_RT");  // =1: runtime-length line kernels only
+    pencil_ct_ = !(pen_rt && pen_rt[0] == '1');
     ks_ = (pen && pen[0] == '1') ? nullptr : find_kernels<T>(m[0], m[1], m[2]);
     hipError_t e = hipSuccess;
     if (ks_) {
@@ -475,6 +508,7 @@ PropArgs<T> PropEngine<T>::args(int nb, const ChainConst& cc) const
     a.n2 = cc.n2;
     a.Nh = cc.Nh;
     a.nb = nb;
+    a.pencil_ct = pencil_ct_ ? 1 : 0;
     a.Ch = Ch_;
     a.Cf = Cf_;
     a.e4A = e4A_;
@@ -516,8 +550,8 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
     if ((e = hipMemsetAsync(gpart_, 0, sizeof(double) * 3 * np, st)) != hipSuccess) return e;
 
     const PropArgs<T> a = args(nb, cc);
-    const PencilTiles p = pencil_tiles(m_[0], m_[1], m_[2]);
-    const size_t lds = pencil_lds_bytes<T>(m_[0], m_[1], m_[2]);
+    const PencilTiles p = pencil_tiles(m_[0], m_[1], m_[2], pencil_ct_);
+    const size_t lds = pencil_lds_bytes<T>(m_[0], m_[1], m_[2], pencil_ct_);
     const double twopi = 8 * atan(1.0);
     const GradConst gc{cc.ds, {twopi / D_[0], twopi / D_[1], twopi / D_[2]}, {D_[0], D_[1], D_[2]}};
     const long long lines = (long long)m_[0] * m_[1];

@@ -97,6 +97,7 @@ private:
     };
     std::vector<GraphEntry> graphs_;
     bool use_graph_ = true;
+    bool pencil_ct_ = true;  // line kernels compiled for the axis length where one exists
     int m_[3] = {0, 0, 0};
     double D_[3] = {0, 0, 0};
     long long M_ = 0;

@@ -92,6 +92,40 @@ __device__ void pc_fft(cpx<T>* buf, int L, int nl, int P, const cpx<T>* tw, int 
     }
 }
 
+// Axis lengths with compile-time kernels: each line pass depends on its own axis length
+// only, so a few lengths cover the large grids (the reference's 200 x 150 x 150 box, FP64
+// 128^3, ...). Their tiles are fixed: nlz = PC_EMAX / 2 / L z lines, t = PC_EMAX / L z
+// columns, and the transform is the compile-time Stockham of fft_engine.hpp (same radix
+// plan and butterflies as pc_fft). Other lengths take the runtime kernels (L = 0).
+#define PC_CT_LENGTHS(X) X(128) X(150) X(160) X(192) X(200) X(256)
+__host__ __device__ constexpr bool pc_ct_len(int L)
+{
+    return L == 128 || L == 150 || L == 160 || L == 192 || L == 200 || L == 256;
+}
+__host__ __device__ constexpr int pc_ct_nlz(int L) { return PC_EMAX / 2 / L > 0 ? PC_EMAX / 2 / L : 1; }
+__host__ __device__ constexpr int pc_ct_t(int L) { return PC_EMAX / L > 0 ? PC_EMAX / L : 1; }
+
+// z-pass transform of nfl fields x nlz lines (runtime, or compile-time for L > 0)
+template <typename T, bool INV, int L>
+__device__ __forceinline__ void pz_fft(cpx<T>* buf, int m2, int nfl, int nlz, int P, const cpx<T>* tw, int tid)
+{
+    if constexpr (L > 0) {
+        constexpr int NZ = pc_ct_nlz(L);
+        if (nfl == 2) fft_lines<T, L, INV, 2 * NZ, 1, pc_pitch(2 * NZ), PC_NT>(buf, tw, tid);
+        else fft_lines<T, L, INV, NZ, 1, pc_pitch(NZ), PC_NT>(buf, tw, tid);
+    } else {
+        pc_fft<T, INV>(buf, m2, nfl * nlz, P, tw, tid);
+    }
+}
+
+// y- or x-pass transform of nl lines of length len (compile-time for L > 0: nl = pc_ct_t(L))
+template <typename T, bool INV, int L>
+__device__ __forceinline__ void pl_fft(cpx<T>* buf, int len, int nl, int P, const cpx<T>* tw, int tid)
+{
+    if constexpr (L > 0) fft_lines<T, L, INV, pc_ct_t(L), 1, pc_pitch(pc_ct_t(L)), PC_NT>(buf, tw, tid);
+    else pc_fft<T, INV>(buf, len, nl, P, tw, tid);
+}
+
 // Element e of a workgroup's tile is handled by thread e % PC_NT in slot e / PC_NT of
 // its registers. Every global load of a kernel goes to registers first, in fully unrolled
 // loops, so all of a thread's loads are in flight together before the first LDS write.
@@ -105,11 +139,12 @@ constexpr int PC_EPT = PC_EMAX / PC_NT;      // in k_py, k_px (lines x length <=
 //   PZ_LAST:  as PZ_STEP without the forward part
 //   PZ_BEGIN: q(0) = 1 stored, Ch <- Fz(E1/4), Cf <- Fz(E1/2) for step 0 (density.h:351-355)
 // LDS: lines [0, nlz) hold Ch, [nlz, 2 nlz) hold Cf, pitch pc_pitch(2 nlz), then m2 twiddles.
-template <typename T>
-__global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, int nlz)
+template <typename T, int L = 0>
+__global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, int nlz_arg)
 {
     extern __shared__ unsigned char smem_raw[];
-    const int m2 = a.m2;
+    const int m2 = L > 0 ? L : a.m2;
+    const int nlz = L > 0 ? pc_ct_nlz(L) : nlz_arg;
     const int nfl = mode == PZ_MID ? 1 : 2;
     const int nlines = nfl * nlz, P = pc_pitch(nlines);
     cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
@@ -145,7 +180,7 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
             }
         }
         __syncthreads();
-        pc_fft<T, true>(buf, m2, nlines, P, tw, tid);
+        pz_fft<T, true, L>(buf, m2, nfl, nlz, P, tw, tid);
     }
 
     if (mode == PZ_MID) {
@@ -161,7 +196,7 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
             }
         }
         __syncthreads();
-        pc_fft<T, false>(buf, m2, nlines, P, tw, tid);
+        pz_fft<T, false, L>(buf, m2, nfl, nlz, P, tw, tid);
 #pragma unroll
         for (int i = 0; i < PZ_EPT; ++i) {
             const int e = tid + i * PC_NT;
@@ -205,7 +240,7 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
     }
     if (mode == PZ_LAST) return;
     __syncthreads();
-    pc_fft<T, false>(buf, m2, nlines, P, tw, tid);
+    pz_fft<T, false, L>(buf, m2, nfl, nlz, P, tw, tid);
 #pragma unroll
     for (int i = 0; i < PZ_EPT; ++i) {
         const int e = tid + i * PC_NT;
@@ -226,11 +261,12 @@ __device__ __forceinline__ cpx<T>* pc_field(const PropArgs<T>& a, int f, int nh)
 
 // y-line pass, forward or inverse, on tz z-columns of one x.
 // grid (m0 * ceil(m2 / tz), nh + nf fields), block PC_NT. LDS: m1 x pc_pitch(tz), m1 twiddles.
-template <typename T, bool INV>
-__global__ void __launch_bounds__(PC_NT) k_py(PropArgs<T> a, int nh, int tz)
+template <typename T, bool INV, int L = 0>
+__global__ void __launch_bounds__(PC_NT) k_py(PropArgs<T> a, int nh, int tz_arg)
 {
     extern __shared__ unsigned char smem_raw[];
-    const int m1 = a.m1, m2 = a.m2, P = pc_pitch(tz);
+    const int m1 = L > 0 ? L : a.m1, m2 = a.m2;
+    const int tz = L > 0 ? pc_ct_t(L) : tz_arg, P = pc_pitch(tz);
     cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
     cpx<T>* tw = buf + (size_t)m1 * P;
     const int tid = threadIdx.x;
@@ -254,7 +290,7 @@ __global__ void __launch_bounds__(PC_NT) k_py(PropArgs<T> a, int nh, int tz)
         if (e < ne) buf[y * P + zz] = r[i];
     }
     __syncthreads();
-    pc_fft<T, INV>(buf, m1, tz, P, tw, tid);
+    pl_fft<T, INV, L>(buf, m1, tz, P, tw, tid);
 #pragma unroll
     for (int i = 0; i < PC_EPT; ++i) {
         const int e = tid + i * PC_NT;
@@ -266,11 +302,12 @@ __global__ void __launch_bounds__(PC_NT) k_py(PropArgs<T> a, int nh, int tz)
 // x-line pass on tx z-columns of one y: forward x, times exp(-k^2 ds/12)/M (Ch) or
 // exp(-k^2 ds/6)/M (Cf), inverse x (density.h:149-156, 198-205; GPU_kernels.h:175-181, 207-213).
 // grid (m1 * ceil(m2 / tx), nh + nf fields), block PC_NT. LDS: m0 x pc_pitch(tx), m0 twiddles.
-template <typename T>
-__global__ void __launch_bounds__(PC_NT) k_px(PropArgs<T> a, int nh, int tx)
+template <typename T, int L = 0>
+__global__ void __launch_bounds__(PC_NT) k_px(PropArgs<T> a, int nh, int tx_arg)
 {
     extern __shared__ unsigned char smem_raw[];
-    const int m0 = a.m0, m2 = a.m2, P = pc_pitch(tx);
+    const int m0 = L > 0 ? L : a.m0, m2 = a.m2;
+    const int tx = L > 0 ? pc_ct_t(L) : tx_arg, P = pc_pitch(tx);
     const size_t sx = (size_t)a.m1 * m2;
     cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
     cpx<T>* tw = buf + (size_t)m0 * P;
@@ -296,7 +333,7 @@ __global__ void __launch_bounds__(PC_NT) k_px(PropArgs<T> a, int nh, int tx)
         if (e < ne) buf[x * P + zz] = r[i];
     }
     __syncthreads();
-    pc_fft<T, false>(buf, m0, tx, P, tw, tid);
+    pl_fft<T, false, L>(buf, m0, tx, P, tw, tid);
     const T gy = a.gy[y];
 #pragma unroll
     for (int i = 0; i < PC_EPT; ++i) {
@@ -306,7 +343,7 @@ __global__ void __launch_bounds__(PC_NT) k_px(PropArgs<T> a, int nh, int tx)
             Kinetic<T>::apply(buf[x * P + zz], a.gx[x], Kinetic<T>::col(gy, a.gz[z0 + zz]), full, a.invM);
     }
     __syncthreads();
-    pc_fft<T, true>(buf, m0, tx, P, tw, tid);
+    pl_fft<T, true, L>(buf, m0, tx, P, tw, tid);
 #pragma unroll
     for (int i = 0; i < PC_EPT; ++i) {
         const int e = tid + i * PC_NT;
@@ -322,8 +359,9 @@ struct PencilTiles {
     int tx;   // z columns per k_px workgroup
 };
 
-// Tiling for an m0 x m1 x m2 grid; every transform stays within PC_EMAX elements.
-inline PencilTiles pencil_tiles(int m0, int m1, int m2)
+// Tiling for an m0 x m1 x m2 grid; every transform stays within PC_EMAX elements. With ct,
+// axes of a compile-time length take that kernel's fixed tile.
+inline PencilTiles pencil_tiles(int m0, int m1, int m2, bool ct)
 {
     auto tile = [](int len, int cap) {
         int t = PC_EMAX / len;
@@ -335,8 +373,8 @@ inline PencilTiles pencil_tiles(int m0, int m1, int m2)
     PencilTiles p;
     p.nlz = PC_EMAX / 2 / m2;
     if (p.nlz < 1) p.nlz = 1;
-    p.ty = tile(m1, m2);
-    p.tx = tile(m0, m2);
+    p.ty = (ct && pc_ct_len(m1)) ? pc_ct_t(m1) : tile(m1, m2);
+    p.tx = (ct && pc_ct_len(m0)) ? pc_ct_t(m0) : tile(m0, m2);
     return p;
 }
 
@@ -360,9 +398,9 @@ constexpr size_t pencil_lds_max()
 
 // dynamic LDS bytes of the three kernels for one grid (what a launch passes)
 template <typename T>
-inline size_t pencil_lds_bytes(int m0, int m1, int m2)
+inline size_t pencil_lds_bytes(int m0, int m1, int m2, bool ct)
 {
-    const PencilTiles p = pencil_tiles(m0, m1, m2);
+    const PencilTiles p = pencil_tiles(m0, m1, m2, ct);
     const size_t z = (size_t)m2 * pc_pitch(2 * p.nlz) + m2;
     const size_t y = (size_t)m1 * pc_pitch(p.ty) + m1;
     const size_t x = (size_t)m0 * pc_pitch(p.tx) + m0;

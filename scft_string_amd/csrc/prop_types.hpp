@@ -12,7 +12,8 @@ struct PropArgs {
     int m0, m1, m2;        // grid (x slowest, z fastest)
     int N, n1, n2, Nh;     // contour steps, block split, homopolymer length
     int nb;                // active batch slots
-    cpx<T>* Ch;            // [slot][M] half-step sandwich work field
+    int pencil_ct;         // line kernels: use the compile-time-length ones where they exist
+    cpx<T>* Ch;           // [slot][M] half-step sandwich work field
     cpx<T>* Cf;            // [slot][M] full-step sandwich work field
     const T* e4A;          // [slot][M] quarter-step factor of species A (Boltz<T> form)
     const T* e4B;          // [slot][M] the same for species B

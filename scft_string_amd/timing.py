@@ -1,0 +1,77 @@
+"""Per-contour-step timing of props on one grid, through the C-ABI.
+
+    python -m scft_string_amd.timing --m 200 150 150 --D 16 12 12 --N 60 --ens 2 \\
+        --phidb 1.16e-5 --f 0.8 --chi 30 --prec fp64 --nrep 1 [--pencil] [--runtime-lengths]
+
+prints one JSON line: ms per contour step (the mean over reps of a props of nrep replicas,
+divided by N), the kernel family, and the algorithmic HBM rate against the 8 TB/s peak.
+--pencil forces the line kernels on grids that also have plane/column kernels
+(SCFT_PENCIL=1); --runtime-lengths forces the runtime-planned line kernels
+(SCFT_PENCIL_RT=1). --fields lamella times a lamellar field state instead of W = 0: the
+step time depends on the data, and zero fields run measurably faster.
+"""
+import argparse
+import json
+import os
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_per_step(m, p):
+    """12 (M p + Mk 2p): 2 chains x 6 real 3-D FFTs, each reading its input and writing its
+    output once, pointwise factors fused (per contour step per replica)."""
+    M = m[0] * m[1] * m[2]
+    Mk = m[0] * m[1] * (m[2] // 2 + 1)
+    return 12 * (M * p + Mk * 2 * p)
+
+
+def lamella_fields(m, D, R, amp=1.5, seed=0):
+    rng = np.random.default_rng(seed)
+    M = int(np.prod(m))
+    x = (np.arange(m[0]) * D[0] / m[0])[:, None, None] * np.ones(m)
+    W = np.zeros((R, 2, M))
+    for i in range(R):
+        W[i, 0] = amp * np.cos(2 * np.pi * x / D[0]).reshape(-1) + 0.1 * rng.standard_normal(M)
+    return W
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--m", type=int, nargs=3, required=True)
+    ap.add_argument("--D", type=float, nargs=3, required=True)
+    ap.add_argument("--N", type=int, required=True)
+    ap.add_argument("--f", type=float, default=0.45)
+    ap.add_argument("--alpha", type=float, default=0.1)
+    ap.add_argument("--ens", type=int, default=1)
+    ap.add_argument("--phidb", type=float, default=0.9)
+    ap.add_argument("--chi", type=float, default=18.0)
+    ap.add_argument("--prec", choices=("fp32", "fp64"), default="fp32")
+    ap.add_argument("--nrep", type=int, default=1)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--pencil", action="store_true")
+    ap.add_argument("--runtime-lengths", action="store_true")
+    ap.add_argument("--fields", choices=("zero", "lamella"), default="zero")
+    a = ap.parse_args(argv)
+    # read by the engine at context creation
+    os.environ["SCFT_PENCIL"] = "1" if a.pencil else "0"
+    os.environ["SCFT_PENCIL_RT"] = "1" if a.runtime_lengths else "0"
+    from .capi import Context
+
+    with Context(tuple(a.m), tuple(a.D), a.N, a.f, a.alpha, a.ens, a.phidb, a.chi, nrep=a.nrep,
+                 precision=a.prec) as ctx:
+        if a.fields == "lamella":
+            ctx.set_fields(lamella_fields(a.m, a.D, a.nrep))
+        ms = ctx.time_props(a.nrep, a.reps)
+        path = ctx.fft_path
+    p = 4 if a.prec == "fp32" else 8
+    gbs = a.nrep * algorithmic_bytes_per_step(a.m, p) / (ms * 1e-3) / 1e9
+    print(json.dumps({"m": a.m, "D": a.D, "N": a.N, "prec": a.prec, "nrep": a.nrep, "path": path,
+                      "runtime_lengths": a.runtime_lengths, "fields": a.fields,
+                      "ms_per_step": round(ms, 4), "alg_GBps": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 3)}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -128,6 +128,30 @@ def test_pencil_and_two_stage_agree(prec, tol, monkeypatch):
     assert dl <= tol and dphi <= 10 * tol
 
 
+@pytest.mark.parametrize("m", [(128, 150, 160), (150, 12, 200)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_compile_time_line_kernels_match_runtime(m, prec, tol, monkeypatch):
+    """Axes of lengths 128, 150, 160, 192, 200, 256 run line kernels compiled for that length
+    (fixed tiles, unrolled passes); SCFT_PENCIL_RT=1 forces the runtime-length plan. Both
+    compute the same transform: props and the box gradient must agree."""
+    D = tuple(0.08 * v for v in m)
+    W = np.stack([_fields(m, D, seed=60 + i) for i in range(2)])
+    out = {}
+    for rt in ("0", "1"):
+        monkeypatch.setenv("SCFT_PENCIL_RT", rt)
+        _say(f"line kernels {m} {prec} runtime={rt}")
+        with Context(m, D, 12, 0.45, 0.1, 1, 0.9, chi=18.0, nrep=2, precision=prec) as ctx:
+            assert ctx.fft_path == "pencil"
+            phi, lnQ, _ = ctx.props_batch(W)
+            _, dq = ctx.dlnq_dbox(W)
+            out[rt] = (phi, lnQ, dq)
+    dl = float(np.max(np.abs(out["0"][1] - out["1"][1])))
+    dphi = _rel(out["0"][0], out["1"][0])
+    dd = _rel(out["0"][2], out["1"][2])
+    _say(f"compile-time vs runtime {m} {prec}: |dlnQ| {dl:.2e}, rel dphi {dphi:.2e}, rel d(dlnQ/dD) {dd:.2e}")
+    assert dl <= tol and dphi <= 10 * tol and dd <= 100 * tol
+
+
 def test_fp64_128_cube_vs_oracle():
     """128^3 FP64: a plane does not fit in LDS, so this grid runs on the line kernels."""
     m, D, N = (128, 128, 128), (8.0, 8.0, 8.0), 16
