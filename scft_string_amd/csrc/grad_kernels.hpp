@@ -137,7 +137,8 @@ __global__ void __launch_bounds__(PC_NT) k_gx(PropArgs<T> a, int mode, int nh, i
     cpx<T>* tw = buf + (size_t)m0 * P;
     const int tid = threadIdx.x;
     const int ntz = (m2 + tx - 1) / tx;
-    const int y = blockIdx.x / ntz, z0 = (blockIdx.x - y * ntz) * tx;
+    const int bt = pc_xcd_block();
+    const int y = bt / ntz, z0 = (bt - y * ntz) * tx;
     const int nz = m2 - z0 < tx ? m2 - z0 : tx;
     const int f = blockIdx.y;
     const bool isB = f >= nh;

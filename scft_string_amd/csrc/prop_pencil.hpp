@@ -126,6 +126,20 @@ __device__ __forceinline__ void pl_fft(cpx<T>* buf, int len, int nl, int P, cons
     else pc_fft<T, INV>(buf, len, nl, P, tw, tid);
 }
 
+// Workgroups go to the 8 XCDs round-robin by linear id, and each XCD has its own L2. The
+// y and x passes renumber blockIdx.x so that each XCD takes one contiguous run of tiles:
+// adjacent z tiles share the cache lines at their boundary and then fetch them once, not
+// once per tile. The map is a bijection on [0, gridDim.x); blocks of equal blockIdx.x % 8
+// share an XCD for every blockIdx.y.
+constexpr int PC_XCD = 8;
+
+__device__ __forceinline__ int pc_xcd_block()
+{
+    const int G = gridDim.x, b = blockIdx.x;
+    const int q = G / PC_XCD, r = G - q * PC_XCD, j = b % PC_XCD;
+    return j * q + (j < r ? j : r) + b / PC_XCD;
+}
+
 // Element e of a workgroup's tile is handled by thread e % PC_NT in slot e / PC_NT of
 // its registers. Every global load of a kernel goes to registers first, in fully unrolled
 // loops, so all of a thread's loads are in flight together before the first LDS write.
@@ -271,7 +285,8 @@ __global__ void __launch_bounds__(PC_NT) k_py(PropArgs<T> a, int nh, int tz_arg)
     cpx<T>* tw = buf + (size_t)m1 * P;
     const int tid = threadIdx.x;
     const int ntz = (m2 + tz - 1) / tz;
-    const int x = blockIdx.x / ntz, z0 = (blockIdx.x - x * ntz) * tz;
+    const int bt = pc_xcd_block();
+    const int x = bt / ntz, z0 = (bt - x * ntz) * tz;
     const int nz = m2 - z0 < tz ? m2 - z0 : tz;
     cpx<T>* g = pc_field(a, blockIdx.y, nh) + (size_t)x * m1 * m2 + z0;
     const int ne = m1 * tz;
@@ -313,7 +328,8 @@ __global__ void __launch_bounds__(PC_NT) k_px(PropArgs<T> a, int nh, int tx_arg)
     cpx<T>* tw = buf + (size_t)m0 * P;
     const int tid = threadIdx.x;
     const int ntz = (m2 + tx - 1) / tx;
-    const int y = blockIdx.x / ntz, z0 = (blockIdx.x - y * ntz) * tx;
+    const int bt = pc_xcd_block();
+    const int y = bt / ntz, z0 = (bt - y * ntz) * tx;
     const int nz = m2 - z0 < tx ? m2 - z0 : tx;
     const bool full = (int)blockIdx.y >= nh;
     cpx<T>* g = pc_field(a, blockIdx.y, nh) + (size_t)y * m2 + z0;

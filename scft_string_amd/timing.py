@@ -2,6 +2,7 @@
 
     python -m scft_string_amd.timing --m 200 150 150 --D 16 12 12 --N 60 --ens 2 \\
         --phidb 1.16e-5 --f 0.8 --chi 30 --prec fp64 --nrep 1 [--pencil] [--runtime-lengths]
+        [--fields lamella] [--copy-probe]
 
 prints one JSON line: ms per contour step (the mean over reps of a props of nrep replicas,
 divided by N), the kernel family, and the algorithmic HBM rate against the 8 TB/s peak.
@@ -53,7 +54,18 @@ def main(argv=None):
     ap.add_argument("--pencil", action="store_true")
     ap.add_argument("--runtime-lengths", action="store_true")
     ap.add_argument("--fields", choices=("zero", "lamella"), default="zero")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first a 512 MiB device copy: the known-size scale for FETCH_SIZE / WRITE_SIZE")
     a = ap.parse_args(argv)
+    if a.copy_probe:
+        import torch
+
+        src = torch.ones(1 << 27, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     # read by the engine at context creation
     os.environ["SCFT_PENCIL"] = "1" if a.pencil else "0"
     os.environ["SCFT_PENCIL_RT"] = "1" if a.runtime_lengths else "0"
