@@ -147,7 +147,7 @@ __global__ void __launch_bounds__(NT) k_plane_end(PropArgs<T> a, int s, int mode
 #pragma unroll
     for (int i = 0; i < PE; ++i) {
         const int e = tid + i * NT;
-        if (NE % NT == 0 || e < NE) qo[e] = q[i];
+        if (NE % NT == 0 || e < NE) store_q(&qo[e], q[i]);
     }
     if (mode == PL_LAST) return;
 
@@ -429,6 +429,31 @@ __global__ void __launch_bounds__(RED_NT) k_scalars(const double* __restrict__ p
 // interior point whatever the parity (A.8).
 __device__ __forceinline__ double wt_odd_first(int s_from_start) { return (s_from_start & 1) ? 4.0 : 2.0; }
 
+// The density sums read each q-store slot once: non-temporal loads.
+template <typename T>
+__device__ __forceinline__ cpx<T> load_q(const cpx<T>* p)
+{
+    using V = T __attribute__((ext_vector_type(2)));
+    const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+    return {v.x, v.y};
+}
+constexpr int DU = 4;  // slot pairs per batch in k_density
+
+// Weights of the copolymer term q1[i] q2[N-i] in the A-block and B-block sums
+// (density.h:413-422): ends 1, the junction n1 in both, interior wt_odd_first from each
+// block's start. A weight of 1 or 0 adds v or exactly nothing, as the branches did.
+struct CoW {
+    double a, b;
+};
+__device__ __forceinline__ CoW co_weight(int i, int n1, int N)
+{
+    if (i == 0) return {1.0, 0.0};
+    if (i < n1) return {wt_odd_first(i), 0.0};
+    if (i == n1) return {1.0, 1.0};
+    if (i < N) return {0.0, wt_odd_first(i - n1)};
+    return {0.0, 1.0};
+}
+
 // Densities (density.h:401-430): phi[rep] = (phiA, phiB, phih, phicB), and block partial
 // sums of (rC, rA, rB, rhA, rhB) for phis (density.h:433-454).
 template <typename T>
@@ -447,37 +472,53 @@ __global__ void __launch_bounds__(RED_NT) k_density(const cpx<T>* __restrict__ q
     double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
     for (long long r = (long long)blockIdx.x * RED_NT + threadIdx.x; r < M; r += (long long)gridDim.x * RED_NT) {
         double cA = 0, cB = 0;
-        // copolymer terms q1[s] q2[N-s], s = 0..N, read in pairs (s, N-s)
-        for (int s = 0; 2 * s <= N; ++s) {
-            const cpx<T> zs = q[(size_t)s * M + r];
-            const cpx<T> zt = q[(size_t)(N - s) * M + r];
-            const int t = N - s;
-            // term at index s
-            {
-                const double v = (double)zs.x * (double)zt.y;
-                if (s == 0) cA += v;
-                else if (s < n1) cA += wt_odd_first(s) * v;
-                else if (s == n1) { cA += v; cB += v; }
-                else if (s < N) cB += wt_odd_first(s - n1) * v;
-                else cB += v;
+        // read in pairs (s, N-s), DU pairs per batch: every load of a batch is in flight
+        // before the first product (the slots are M apart, so nothing else orders them)
+        const int S = N / 2;  // s = 0..S
+        int s = 0;
+        for (; s + DU - 1 <= S; s += DU) {
+            cpx<T> zs[DU], zt[DU];
+#pragma unroll
+            for (int u = 0; u < DU; ++u) {
+                zs[u] = load_q(q + (size_t)(s + u) * M + r);
+                zt[u] = load_q(q + (size_t)(N - s - u) * M + r);
             }
-            if (t != s) {
-                const double v = (double)zt.x * (double)zs.y;
-                if (t == 0) cA += v;
-                else if (t < n1) cA += wt_odd_first(t) * v;
-                else if (t == n1) { cA += v; cB += v; }
-                else if (t < N) cB += wt_odd_first(t - n1) * v;
-                else cB += v;
+#pragma unroll
+            for (int u = 0; u < DU; ++u) {
+                const int si = s + u, ti = N - s - u;
+                const double v = (double)zs[u].x * (double)zt[u].y;
+                const CoW w = co_weight(si, n1, N);
+                cA += w.a * v;
+                cB += w.b * v;
+                if (ti != si) {
+                    const double v2 = (double)zt[u].x * (double)zs[u].y;
+                    const CoW w2 = co_weight(ti, n1, N);
+                    cA += w2.a * v2;
+                    cB += w2.b * v2;
+                }
+            }
+        }
+        for (; s <= S; ++s) {
+            const cpx<T> zs = load_q(q + (size_t)s * M + r), zt = load_q(q + (size_t)(N - s) * M + r);
+            const double v = (double)zs.x * (double)zt.y;
+            const CoW w = co_weight(s, n1, N);
+            cA += w.a * v;
+            cB += w.b * v;
+            if (N - s != s) {
+                const double v2 = (double)zt.x * (double)zs.y;
+                const CoW w2 = co_weight(N - s, n1, N);
+                cA += w2.a * v2;
+                cB += w2.b * v2;
             }
         }
         // homopolymers: B = 2 q2[Nh] + sum_{s=1}^{Nh-1} w_s q2[s] q2[Nh-s] (density.h:409-412),
         // A the same over q1 (density.h:404-407), from the same slots' real parts
-        const cpx<T> zh = q[(size_t)Nh * M + r];
+        const cpx<T> zh = load_q(q + (size_t)Nh * M + r);
         double hB = 2.0 * (double)zh.y, hA = 2.0 * (double)zh.x;
         for (int s = 1; 2 * s <= Nh; ++s) {
             const int t = Nh - s;
             if (t < 1) break;
-            const cpx<T> zs = q[(size_t)s * M + r], zt = q[(size_t)t * M + r];
+            const cpx<T> zs = load_q(q + (size_t)s * M + r), zt = load_q(q + (size_t)t * M + r);
             const double w = (t != s) ? wt_odd_first(s) + wt_odd_first(t) : wt_odd_first(s);
             hB += w * ((double)zs.y * (double)zt.y);
             hA += w * ((double)zs.x * (double)zt.x);

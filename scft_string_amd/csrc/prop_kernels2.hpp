@@ -364,7 +364,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT
         const int w = tid + it * NT;
         if (P::z1_valid(w))
 #pragma unroll
-            for (int a2 = 0; a2 < A2; ++a2) qo[P::z1_elem(w, a2)] = q[it][a2];
+            for (int a2 = 0; a2 < A2; ++a2) store_q(&qo[P::z1_elem(w, a2)], q[it][a2]);
     }
     if (mode == PL_LAST) return;
 

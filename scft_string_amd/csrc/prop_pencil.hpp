@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
                     q.x = Boltz<T>::rich(h0.x, h1.x, qa ? va[i] : vb[i]);
                     q.y = Boltz<T>::rich(h0.y, h1.y, pb ? vb[i] : va[i]);
                 }
-                qo[e] = q;
+                store_q(&qo[e], q);
                 if (mode != PZ_LAST) {
                     const T ex = na ? va[i] : vb[i], ey = nbb ? vb[i] : va[i];
                     buf[z * P + l] = {Boltz<T>::mul4(q.x, ex), Boltz<T>::mul4(q.y, ey)};

@@ -27,6 +27,17 @@ struct PropArgs {
     T invM;
 };
 
+// Store to the q store: written once per contour step and read only by the density sums
+// after the contour loop. A non-temporal store keeps this stream (N+1 fields per props)
+// from displacing the step's working fields (Ch, Cf, factors) from the caches.
+template <typename T>
+__device__ __forceinline__ void store_q(cpx<T>* p, cpx<T> v)
+{
+    using V = T __attribute__((ext_vector_type(2)));
+    const V w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<V*>(p));
+}
+
 // Per-slot scalars (density.h:376-399) in FP64 from the factored sums:
 // scal[slot][SCAL_N] = {lnQ, sC, sH, QC, Q2, Q1, QC', Q2', sHA, Q1'}. sC, sH, sHA multiply
 // the copolymer, B-homopolymer and A-homopolymer Simpson sums directly; primes are the
