@@ -156,6 +156,13 @@ scft_status scft_dlnq_dbox(scft_ctx *ctx, const double *W, int nrow, double *lnQ
  * the reference. */
 scft_status scft_box_stress(scft_ctx *ctx, double *dFdD, double *dFdchi);
 
+/* Backward pass of the replicas' free energies F_r (FEs of FreeE, scft.cu:812-853) at the
+ * current saddle fields, as an autograd backward takes it: from dL_dF[nrep] it returns
+ * dL_dD[nrep][3] = dL_dF[r] dF_r/dD_i and dL_dchi[nrep] = dL_dF[r] dF_r/dchiN (sum over r
+ * for the shared box and chiN). The derivatives are those of scft_box_stress. dL_dF must
+ * not be NULL; either output may be. No counterpart in the reference. */
+scft_status scft_backward(scft_ctx *ctx, const double *dL_dF, double *dL_dD, double *dL_dchi);
+
 /* New box lengths D[3] for every later call: rebuilds the kinetic tables (setup_k_sq,
  * density.h:79-95, which the reference runs once at start-up). */
 scft_status scft_set_box(scft_ctx *ctx, const double D[3]);

@@ -21,6 +21,7 @@ EXPORTED = [
     "scft_get_tangent", "scft_solve_w_plus", "scft_string_iterate", "scft_free_energy",
     "scft_debug_qq_invariant", "scft_debug_time_props", "scft_fft_path", "scft_dlnq_dbox",
     "scft_box_stress", "scft_set_box", "scft_relax_box", "scft_iterate_concurrent", "scft_engine_info",
+    "scft_backward",
 ]
 _STRING_RESULT = ("scft_last_error", "scft_fft_path", "scft_engine_info")
 
@@ -83,6 +84,7 @@ def lib():
         L.scft_engine_info.restype = C.c_char_p
         L.scft_dlnq_dbox.argtypes = [vp, dp, C.c_int, dp, dp]
         L.scft_box_stress.argtypes = [vp, dp, dp]
+        L.scft_backward.argtypes = [vp, dp, dp, dp]
         L.scft_set_box.argtypes = [vp, dp]
         L.scft_relax_box.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp, ip]
         L.scft_iterate_concurrent.argtypes = [C.POINTER(vp), C.c_int, ip, C.c_double, C.POINTER(dp),
@@ -261,6 +263,15 @@ class Context:
         dFdchi = np.empty(self.nrep)
         self._chk(lib().scft_box_stress(self._h, _dp(dFdD), _dp(dFdchi)))
         return dFdD, dFdchi
+
+    def backward(self, dL_dF):
+        """scft_backward: dL/dD (nrep, 3) and dL/dchiN (nrep,) from dL/dF (nrep,) at the
+        current field state."""
+        g = np.ascontiguousarray(dL_dF, dtype=np.float64).reshape(self.nrep)
+        dD = np.empty((self.nrep, 3))
+        dchi = np.empty(self.nrep)
+        self._chk(lib().scft_backward(self._h, _dp(g), _dp(dD), _dp(dchi)))
+        return dD, dchi
 
     def set_box(self, D):
         D = np.ascontiguousarray(D, dtype=np.float64).reshape(3)

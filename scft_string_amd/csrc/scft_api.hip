@@ -855,6 +855,23 @@ scft_status scft_box_stress(scft_ctx* c, double* dFdD, double* dFdchi)
     return SCFT_OK;
 }
 
+scft_status scft_backward(scft_ctx* c, const double* dL_dF, double* dL_dD, double* dL_dchi)
+{
+    if (!c) return SCFT_ERR_INVALID;
+    if (!dL_dF) return fail(c, SCFT_ERR_INVALID, "scft_backward: dL_dF is NULL");
+    const int R = c->R;
+    for (int r = 0; r < R; ++r)
+        if (!std::isfinite(dL_dF[r])) return fail(c, SCFT_ERR_INVALID, "scft_backward: dL_dF is not finite");
+    std::vector<double> dD(dL_dD ? 3 * (size_t)R : 0), dchi(dL_dchi ? (size_t)R : 0);
+    SCFTCHK(scft_box_stress(c, dL_dD ? dD.data() : nullptr, dL_dchi ? dchi.data() : nullptr));
+    for (int r = 0; r < R; ++r) {
+        if (dL_dD)
+            for (int i = 0; i < 3; ++i) dL_dD[3 * r + i] = dL_dF[r] * dD[3 * r + i];
+        if (dL_dchi) dL_dchi[r] = dL_dF[r] * dchi[r];
+    }
+    return SCFT_OK;
+}
+
 scft_status scft_set_box(scft_ctx* c, const double D[3])
 {
     if (!c || !D || !(D[0] > 0 && D[1] > 0 && D[2] > 0))

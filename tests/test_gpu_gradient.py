@@ -106,10 +106,9 @@ def _lamella(D0):
     return np.stack([2.0 * np.cos(2 * np.pi * x / D0), np.zeros(32)])
 
 
-def test_box_stress_at_saddle_matches_oracle():
-    """scft_box_stress at oracle-converged fields: -d lnQ/dD and <phiA phiB> at 1e-10."""
+def _saddle(D):
+    """Oracle-converged lamella at box D: fields W (2, M), last props r, grid g."""
     c = LAM
-    D = (4.0, 1.0, 1.0)
     g = orc.Grid(c["m"], D)
     W = _lamella(D[0])
     for _ in range(20000):
@@ -120,6 +119,14 @@ def test_box_stress_at_saddle_matches_oracle():
             break
         W[0] += 0.1 * dm
         W[1] += 0.05 * dp
+    return W, r, g
+
+
+def test_box_stress_at_saddle_matches_oracle():
+    """scft_box_stress at oracle-converged fields: -d lnQ/dD and <phiA phiB> at 1e-10."""
+    c = LAM
+    D = (4.0, 1.0, 1.0)
+    W, r, g = _saddle(D)
     _, dl = g.dlnq_dbox(W, c["N"], c["alpha"], c["phidb"], c["f"], c["ens"])
     with Context(c["m"], D, c["N"], c["f"], c["alpha"], c["ens"], c["phidb"], c["chi"], dostring=0) as ctx:
         ctx.set_fields(W[None])
@@ -144,3 +151,44 @@ def test_relax_box_reaches_zero_stress():
     _, dl = orc.Grid(c["m"], D).dlnq_dbox(W.reshape(2, -1), c["N"], c["alpha"], c["phidb"], c["f"], c["ens"])
     assert abs(-dl[0] - g[0]) <= 1e-10 * max(abs(dl[0]), 1e-3)
     assert 4.3 < D[0] < 4.7
+
+
+def test_autograd_backward_of_free_energy():
+    """FreeEnergy.apply(ctx, D, chi) and its backward (scft_backward): forward gives FreeE's
+    FEs; D.grad and chi.grad are the dL/dF-weighted sums of scft_box_stress over replicas.
+    Two replicas: the saddle and its translate by a quarter period (also a saddle, same F,
+    same stress). The stress itself is pinned against the oracle above."""
+    torch = pytest.importorskip("torch")
+    from scft_string_amd.autograd import FreeEnergy
+
+    c = LAM
+    D = (4.0, 1.0, 1.0)
+    W, r, g = _saddle(D)
+    W2 = np.stack([np.roll(W[0], 8), np.roll(W[1], 8)])
+    _, dl = g.dlnq_dbox(W, c["N"], c["alpha"], c["phidb"], c["f"], c["ens"])
+    wts = (0.3, 1.7)
+    with Context(c["m"], D, c["N"], c["f"], c["alpha"], c["ens"], c["phidb"], c["chi"], nrep=2,
+                 dostring=0) as ctx:
+        ctx.set_fields(np.stack([W, W2]))
+        dFdD, dFdchi = ctx.box_stress()
+        Dt = torch.tensor(D, dtype=torch.float64, requires_grad=True)
+        chit = torch.tensor(c["chi"], dtype=torch.float64, requires_grad=True)
+        F = FreeEnergy.apply(ctx, Dt, chit)
+        (torch.tensor(wts, dtype=torch.float64) * F).sum().backward()
+        fe = ctx.free_energy()["FEs"]
+        dD1, dchi1 = ctx.backward(np.array([1.0, 0.0]))
+        with pytest.raises(ValueError):
+            FreeEnergy.apply(ctx, Dt.detach() * 1.01, chit)
+    Fh = F.detach().numpy()
+    _say(f"autograd: F {Fh}, D.grad {Dt.grad.numpy()}, chi.grad {chit.grad.item()!r}")
+    assert np.allclose(Fh, fe, rtol=1e-13, atol=0)
+    assert abs(Fh[0] - Fh[1]) <= 1e-10 * abs(Fh[0])
+    expD = wts[0] * dFdD[0] + wts[1] * dFdD[1]
+    assert _rel(Dt.grad.numpy(), expD) <= 1e-12
+    expchi = wts[0] * dFdchi[0] + wts[1] * dFdchi[1]
+    assert abs(chit.grad.item() - expchi) <= 1e-12 * abs(expchi)
+    # free_energy, like FreeE, first relaxes W+ by solve_field0 (scft.cu:817-818): at this
+    # saddle that moves the fields, and the stress after it, by rounding only
+    assert _rel(dD1[0], dFdD[0]) <= 1e-10 and np.all(dD1[1] == 0) and dchi1[1] == 0
+    # sum_r dF_r/dD_x: both replicas carry the oracle's stress -d lnQ/dD_x
+    assert _rel(Dt.grad.numpy()[0], -(wts[0] + wts[1]) * dl[0]) <= 1e-10
