@@ -74,6 +74,12 @@ typedef struct scft_params {
                                 Stockham, line kernels) x replica-group streams (1-3) x graph
                                 replay on that batch and keeps the fastest. FP32 results then
                                 depend on the choice (the families round differently). */
+    int ckpt;                /* 0: keep the propagators of every contour index (2(N+1) fields
+                                per replica, as density.h:293). K > 0: keep every K-th index
+                                (K ~ sqrt(N)) and recompute the rest in windows of K for the
+                                densities and the box gradient: N/K + 2K + 5 fields per
+                                replica, about twice the propagation work. Results agree with
+                                K = 0 to rounding. */
 } scft_params;
 
 typedef struct scft_ctx scft_ctx;

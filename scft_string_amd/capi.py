@@ -39,7 +39,7 @@ class Params(C.Structure):
         ("nrep", C.c_int), ("precision", C.c_int), ("fix", C.c_int * 2), ("dostring", C.c_int),
         ("justFE", C.c_int), ("springsteps", C.c_int), ("device", C.c_int), ("fe_fp64", C.c_int),
         ("phihA", C.c_double), ("anderson", C.c_int), ("anderson_lambda", C.c_double),
-        ("anderson_start", C.c_double), ("autotune", C.c_int),
+        ("anderson_start", C.c_double), ("autotune", C.c_int), ("ckpt", C.c_int),
     ]
 
 
@@ -106,7 +106,7 @@ class Context:
 
     def __init__(self, m, D, N, f, alpha, ens, phidb, chi, nrep=1, precision="fp64", fix=(0, 0),
                  dostring=1, justFE=0, springsteps=0, device=0, fe_fp64=1, phihA=0.0, anderson=0,
-                 anderson_lambda=0.0, anderson_start=0.0, autotune=0):
+                 anderson_lambda=0.0, anderson_start=0.0, autotune=0, ckpt=0):
         self.m = tuple(int(v) for v in m)
         self.D = tuple(float(v) for v in D)
         self.M = int(np.prod(self.m))
@@ -118,7 +118,7 @@ class Context:
                    precision=prec, fix=(C.c_int * 2)(*fix), dostring=int(dostring), justFE=int(justFE),
                    springsteps=int(springsteps), device=int(device), fe_fp64=int(fe_fp64),
                    phihA=float(phihA), anderson=int(anderson), anderson_lambda=float(anderson_lambda),
-                   anderson_start=float(anderson_start), autotune=int(autotune))
+                   anderson_start=float(anderson_start), autotune=int(autotune), ckpt=int(ckpt))
         self.params = p
         h = C.c_void_p()
         rc = lib().scft_ctx_create(C.byref(p), C.byref(h))

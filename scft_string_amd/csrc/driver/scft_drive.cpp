@@ -108,6 +108,7 @@ Run set_parameters(const Params& P)
     get("phihA", p.phihA);
     geti("anderson", p.anderson);
     geti("autotune", p.autotune);
+    geti("ckpt", p.ckpt);
     get("anderson_lambda", p.anderson_lambda);
     get("anderson_start", p.anderson_start);
     geti("relaxbox", r.relaxbox);

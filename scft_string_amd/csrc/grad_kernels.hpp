@@ -54,8 +54,8 @@ __global__ void __launch_bounds__(PC_NT) k_gz(PropArgs<T> a, int mode, int sb, i
     T ev[PZ_EPT];
     cpx<T> c0[PZ_EPT];
     if (mode == GZ_INIT) {
-        const cpx<T>* qb = a.qs + ((size_t)slot * (size_t)(a.N + 1) + (size_t)sb) * (size_t)a.M + (size_t)l0 * m2;
-        const cpx<T>* qa = a.qs + ((size_t)slot * (size_t)(a.N + 1) + (size_t)sa) * (size_t)a.M + (size_t)l0 * m2;
+        const cpx<T>* qb = a.qs + ((size_t)slot * (size_t)a.qpitch + (size_t)sb) * (size_t)a.M + (size_t)l0 * m2;
+        const cpx<T>* qa = a.qs + ((size_t)slot * (size_t)a.qpitch + (size_t)sa) * (size_t)a.M + (size_t)l0 * m2;
 #pragma unroll
         for (int i = 0; i < PZ_EPT; ++i) {
             const int e = tid + i * PC_NT;

@@ -67,6 +67,12 @@ struct KS {
                                (int)(s == a.N - 1 ? PL_LAST : PL_STEP));
         }
     }
+    static void resume_legacy(const PropArgs<T>& a, int c, hipStream_t st)
+    {
+        if constexpr (legacy_ok)
+            hipLaunchKernelGGL((k_plane_end<T, N1, N2, NTP>), dim3(N0, a.nb), dim3(NTP), lds_plane, st, a, c,
+                               (int)PL_RESUME);
+    }
     static hipError_t setup_two()
     {
         if constexpr (two_ok) {
@@ -100,18 +106,27 @@ struct KS {
                                (int)(s == a.N - 1 ? PL_LAST : PL_STEP));
         }
     }
+    static void resume_two(const PropArgs<T>& a, int c, hipStream_t st)
+    {
+        if constexpr (two_ok)
+            hipLaunchKernelGGL((k_plane_end2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, c,
+                               (int)PL_RESUME);
+    }
     static KernelSet<T> make()
     {
-        KernelSet<T> k{N0, N1, N2, &setup_legacy, &begin_legacy, &step_legacy, nullptr, nullptr, nullptr, "stockham"};
+        KernelSet<T> k{N0,      N1,      N2,      &setup_legacy, &begin_legacy,  &step_legacy,
+                       nullptr, nullptr, nullptr, "stockham",    &resume_legacy, nullptr};
         if (two_ok) {
             k.setup2 = &setup_two;
             k.begin2 = &begin_two;
             k.step2 = &step_two;
+            k.resume2 = &resume_two;
         }
         if (!legacy_ok) {
             k.setup = &setup_two;
             k.begin = &begin_two;
             k.step = &step_two;
+            k.resume = &resume_two;
         }
         return k;
     }
@@ -227,6 +242,15 @@ static void pencil_begin(const PropArgs<T>& a, hipStream_t st)
 }
 
 template <typename T>
+static void pencil_resume(const PropArgs<T>& a, int c, hipStream_t st)
+{
+    const PencilTiles p = pencil_tiles(a.m0, a.m1, a.m2, a.pencil_ct);
+    const size_t lds = pencil_lds_bytes<T>(a.m0, a.m1, a.m2, a.pencil_ct);
+    pencil_pz<T>(a, c, PZ_RESUME, p, lds, st);
+    pencil_py<T, false>(a, a.nb, 2 * a.nb, p, lds, st);
+}
+
+template <typename T>
 static void pencil_step(const PropArgs<T>& a, int s, hipStream_t st)
 {
     const PencilTiles p = pencil_tiles(a.m0, a.m1, a.m2, a.pencil_ct);
@@ -296,11 +320,13 @@ static hipError_t upload_twiddles(cpx<T>** dst, int n)
 }
 
 template <typename T>
-std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nbmax)
+std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nbmax, int ck)
 {
     for (int i = 0; i < 3; ++i) { m_[i] = m[i]; D_[i] = D[i]; }
     N_ = N;
     nbmax_ = nbmax;
+    ck_ = ck > 0 ? ck : 0;
+    qpitch_ = ck_ ? ck_pitch(N, ck_) : N + 1;
     M_ = (long long)m[0] * m[1] * m[2];
     // Plane/column kernels where this grid has them; the line kernels otherwise, or when
     // SCFT_PENCIL=1 is set.
@@ -315,11 +341,13 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
         setup_ = use_two ? ks_->setup2 : ks_->setup;
         begin_ = use_two ? ks_->begin2 : ks_->begin;
         step_ = use_two ? ks_->step2 : ks_->step;
+        resume_ = use_two ? ks_->resume2 : ks_->resume;
         path_ = use_two ? "two-stage" : "stockham";
         e = setup_();
     } else if (pencil_supported(m[0], m[1], m[2])) {
         begin_ = &pencil_begin<T>;
         step_ = &pencil_step<T>;
+        resume_ = &pencil_resume<T>;
         path_ = "pencil";
         e = pencil_setup<T>();
     } else {
@@ -339,7 +367,7 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
     if (!alloc((void**)&Ch_, fld) || !alloc((void**)&Cf_, fld) ||
         !alloc((void**)&e4A_, sizeof(T) * (size_t)M_ * nbmax) ||
         !alloc((void**)&e4B_, sizeof(T) * (size_t)M_ * nbmax) ||
-        !alloc((void**)&qs_, sizeof(cpx<T>) * (size_t)M_ * (size_t)(N + 1) * nbmax) ||
+        !alloc((void**)&qs_, sizeof(cpx<T>) * (size_t)M_ * (size_t)qpitch_ * nbmax) ||
         !alloc((void**)&part_, sizeof(double) * (size_t)nblk_ * 5 * nbmax) ||
         !alloc((void**)&cAB_, sizeof(double) * 2 * nbmax) || !alloc((void**)&scal_, sizeof(double) * SCAL_N * nbmax))
         return "device allocation failed (propagator buffers)";
@@ -365,10 +393,11 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
             return "stream / event creation failed";
     // kernel families this grid can run, for the autotuner
     fams_.clear();
-    if (ks_ && ks_->step2) fams_.push_back({"two-stage", ks_->setup2, ks_->begin2, ks_->step2});
-    if (ks_ && ks_->step != ks_->step2) fams_.push_back({"stockham", ks_->setup, ks_->begin, ks_->step});
+    if (ks_ && ks_->step2) fams_.push_back({"two-stage", ks_->setup2, ks_->begin2, ks_->step2, ks_->resume2});
+    if (ks_ && ks_->step != ks_->step2)
+        fams_.push_back({"stockham", ks_->setup, ks_->begin, ks_->step, ks_->resume});
     if (pencil_supported(m[0], m[1], m[2]))
-        fams_.push_back({"pencil", &pencil_setup<T>, &pencil_begin<T>, &pencil_step<T>});
+        fams_.push_back({"pencil", &pencil_setup<T>, &pencil_begin<T>, &pencil_step<T>, &pencil_resume<T>});
     return "";
 }
 
@@ -396,6 +425,7 @@ hipError_t PropEngine<T>::tune(const PropArgs<T>& a, int nb, const ChainConst& c
             for (int gph = 0; gph < 2 && e == hipSuccess; ++gph) {
                 begin_ = fams_[f].begin;
                 step_ = fams_[f].step;
+                resume_ = fams_[f].resume;
                 nstreams_ = ns;
                 use_graph_ = gph == 1;
                 for (auto& g : graphs_) (void)hipGraphExecDestroy(g.exec);
@@ -427,6 +457,7 @@ hipError_t PropEngine<T>::tune(const PropArgs<T>& a, int nb, const ChainConst& c
     if ((e = fams_[best.fam].setup()) != hipSuccess) return e;
     begin_ = fams_[best.fam].begin;
     step_ = fams_[best.fam].step;
+    resume_ = fams_[best.fam].resume;
     path_ = fams_[best.fam].name;
     nstreams_ = best.streams;
     use_graph_ = best.graph;
@@ -439,9 +470,12 @@ hipError_t PropEngine<T>::tune(const PropArgs<T>& a, int nb, const ChainConst& c
 template <typename T>
 std::string PropEngine<T>::config() const
 {
-    char b[160];
-    snprintf(b, sizeof b, "%s, %d replica-group stream%s, %s%s", path_, nstreams_, nstreams_ > 1 ? "s" : "",
-             use_graph_ ? "graph replay" : "direct launches", tuned_ ? " (autotuned)" : "");
+    char b[224];
+    int n = snprintf(b, sizeof b, "%s, %d replica-group stream%s, %s%s", path_, nstreams_, nstreams_ > 1 ? "s" : "",
+                     use_graph_ ? "graph replay" : "direct launches", tuned_ ? " (autotuned)" : "");
+    if (ck_ && n > 0 && n < (int)sizeof b)
+        snprintf(b + n, sizeof b - n, ", q checkpointed every %d steps (%d of %d entries per replica)", ck_, qpitch_,
+                 N_ + 1);
     return b;
 }
 
@@ -487,11 +521,58 @@ hipError_t PropEngine<T>::run(const double* dW, long long wstride, double* dphi,
     hipError_t e = contour(a, nb, cc, st);
     if (e != hipSuccess) return e;
     if (timing_) (void)hipEventRecord(ev_[1], st);
-    hipLaunchKernelGGL((k_qsum<T>), rg, dim3(RED_NT), 0, st, qs_, M_, cc.N, cc.Nh, part_);
+    hipLaunchKernelGGL((k_qsum<T>), rg, dim3(RED_NT), 0, st, qs_, M_, qpitch_, store_index(cc.N, cc),
+                       store_index(cc.Nh, cc), part_);
     hipLaunchKernelGGL(k_scalars, dim3(nb), dim3(RED_NT), 0, st, part_, nblk_, M_, cc, cAB_, scal_);
-    hipLaunchKernelGGL((k_density<T>), rg, dim3(RED_NT), 0, st, qs_, M_, cc, scal_, dphi, phistride, d_slot2rep,
-                       part_);
+    if (ck_ == 0) {
+        hipLaunchKernelGGL((k_density<T>), rg, dim3(RED_NT), 0, st, qs_, M_, cc, scal_, dphi, phistride,
+                           d_slot2rep, part_);
+    } else {
+        e = windowed_density(a, nb, cc, dphi, phistride, d_slot2rep, st);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k_phis, dim3(nb), dim3(RED_NT), 0, st, part_, nblk_, M_, d_phis);
+    return hipGetLastError();
+}
+
+// Recompute of contour indices lo..hi (hi <= N) into window w of a checkpointed store:
+// resume from the checkpoint c at or below lo, then steps c .. hi-1. Only lo..hi are stored.
+template <typename T>
+void PropEngine<T>::window(PropArgs<T> a, int w, int lo, int hi, hipStream_t st)
+{
+    const int c = (lo / ck_) * ck_;
+    a.qlo = lo;
+    a.qhi = hi;
+    a.qbase = ck_window(N_, ck_, w);
+    a.qsrc = c / ck_;
+    resume_(a, c, st);
+    for (int s = c; s < hi; ++s) step_(a, s, st);
+}
+
+// The Simpson sums of k_density on a checkpointed store. Pairs (s, L - s) with L = N
+// (copolymer) or Nh (homopolymer) and s <= L - s are taken K lower indices at a time:
+// window 0 holds s in [lo, hi), window 1 the partners L - s in (L - hi, L - lo].
+template <typename T>
+hipError_t PropEngine<T>::windowed_density(const PropArgs<T>& a, int nb, const ChainConst& cc, double* dphi,
+                                           long long phistride, const int* d_slot2rep, hipStream_t st)
+{
+    const dim3 rg(nblk_, nb);
+    hipLaunchKernelGGL(k_dens_zero, rg, dim3(RED_NT), 0, st, M_, dphi, phistride, d_slot2rep);
+    const int K = ck_;
+    for (int kind = 0; kind < 2; ++kind) {
+        const int L = kind == 0 ? cc.N : cc.Nh;
+        const int first = kind == 0 ? 0 : 1;  // homopolymer pairs start at s = 1
+        for (int lo = first; 2 * lo <= L && (kind == 0 || L - lo >= 1); lo += K) {
+            int hi = std::min(lo + K, L / 2 + 1);
+            if (hi <= lo) break;
+            window(a, 0, lo, hi - 1, st);
+            window(a, 1, L - hi + 1, L - lo, st);
+            hipLaunchKernelGGL((k_dens_win<T>), rg, dim3(RED_NT), 0, st, qs_, M_, qpitch_, cc, kind, lo, hi, lo,
+                               ck_window(N_, K, 0), L - hi + 1, ck_window(N_, K, 1), dphi, phistride, d_slot2rep);
+        }
+    }
+    hipLaunchKernelGGL((k_dens_finish<T>), rg, dim3(RED_NT), 0, st, qs_, M_, qpitch_, store_index(cc.Nh, cc), scal_,
+                       dphi, phistride, d_slot2rep, part_);
     return hipGetLastError();
 }
 
@@ -514,6 +595,12 @@ PropArgs<T> PropEngine<T>::args(int nb, const ChainConst& cc) const
     a.e4A = e4A_;
     a.e4B = e4B_;
     a.qs = qs_;
+    a.qpitch = qpitch_;
+    a.qck = ck_;
+    a.qlo = 1;  // no window
+    a.qhi = 0;
+    a.qbase = 0;
+    a.qsrc = 0;
     a.tw0 = tw_[0];
     a.tw1 = tw_[1];
     a.tw2 = tw_[2];
@@ -557,28 +644,51 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
     const long long lines = (long long)m_[0] * m_[1];
     const dim3 gzg((unsigned)((lines + p.nlz - 1) / p.nlz), nb);
     const int ntx = (m_[2] + p.tx - 1) / p.tx;
-    for (int chain = 0; chain < 3; ++chain) {
-        // chain 2 (the A homopolymer's Q1) only when it is present
-        const int ns = chain == 0 ? cc.N : (chain == 2 && cc.phihA == 0.0 ? 0 : cc.Nh);
+    // The term of contour step s of a chain, b and a read from q-store entries ib, ia:
+    //   chain 0: b = q1[s], a = q2[N-1-s], species of chain 1 on step s (density.h:136-146)
+    //   chain 1: b = q2[t], a = q2[Nh-1-t], species of chain 2 on step t
+    //   chain 2: b = q1[t], a = q1[Nh-1-t], species of chain 1 on step t (A for Nh <= n1)
+    auto term = [&](int chain, int s, int ib, int ia) {
         double* pc = gpart_ + (size_t)chain * np;
-        for (int s = 0; s < ns; ++s) {
-            // chain 0: b = q1[s], a = q2[N-1-s], species of chain 1 on step s (density.h:136-146)
-            // chain 1: b = q2[t], a = q2[Nh-1-t], species of chain 2 on step t
-            // chain 2: b = q1[t], a = q1[Nh-1-t], species of chain 1 on step t (A for Nh <= n1)
-            const int sa = chain == 0 ? cc.N - 1 - s : cc.Nh - 1 - s;
-            const int pbk = chain == 1 ? 1 : 0, pak = chain == 2 ? 0 : 1;
-            const int spec = chain == 1 ? (s < cc.n2 ? 1 : 0) : (s < cc.n1 ? 0 : 1);
-            hipLaunchKernelGGL((k_gz<T>), gzg, dim3(PC_NT), lds, st, a, (int)GZ_INIT, s, pbk, sa, pak, spec, p.nlz);
-            pencil_py<T, false>(a, nb, 2 * nb, p, lds, st);
-            hipLaunchKernelGGL((k_gx<T>), dim3(m_[1] * ntx, 2 * nb), dim3(PC_NT), lds, st, a, (int)GX_SPLIT, nb,
-                               p.tx, K0_, Kf_);
-            pencil_py<T, true>(a, nb, nb, p, lds, st);
-            hipLaunchKernelGGL((k_gz<T>), gzg, dim3(PC_NT), lds, st, a, (int)GZ_MID, 0, 0, 0, 0, spec, p.nlz);
-            pencil_py<T, false>(a, nb, nb, p, lds, st);
-            hipLaunchKernelGGL((k_gx<T>), dim3(m_[1] * ntx, nb), dim3(PC_NT), lds, st, a, (int)GX_FWD, nb, p.tx,
-                               K1_, (cpx<T>*)nullptr);
-            hipLaunchKernelGGL((k_gacc<T>), dim3(nblk_, nb), dim3(RED_NT), 0, st, K0_, K1_, Kf_, M_, m_[0], m_[1],
-                               m_[2], gc, pc);
+        const int pbk = chain == 1 ? 1 : 0, pak = chain == 2 ? 0 : 1;
+        const int spec = chain == 1 ? (s < cc.n2 ? 1 : 0) : (s < cc.n1 ? 0 : 1);
+        hipLaunchKernelGGL((k_gz<T>), gzg, dim3(PC_NT), lds, st, a, (int)GZ_INIT, ib, pbk, ia, pak, spec, p.nlz);
+        pencil_py<T, false>(a, nb, 2 * nb, p, lds, st);
+        hipLaunchKernelGGL((k_gx<T>), dim3(m_[1] * ntx, 2 * nb), dim3(PC_NT), lds, st, a, (int)GX_SPLIT, nb, p.tx,
+                           K0_, Kf_);
+        pencil_py<T, true>(a, nb, nb, p, lds, st);
+        hipLaunchKernelGGL((k_gz<T>), gzg, dim3(PC_NT), lds, st, a, (int)GZ_MID, 0, 0, 0, 0, spec, p.nlz);
+        pencil_py<T, false>(a, nb, nb, p, lds, st);
+        hipLaunchKernelGGL((k_gx<T>), dim3(m_[1] * ntx, nb), dim3(PC_NT), lds, st, a, (int)GX_FWD, nb, p.tx, K1_,
+                           (cpx<T>*)nullptr);
+        hipLaunchKernelGGL((k_gacc<T>), dim3(nblk_, nb), dim3(RED_NT), 0, st, K0_, K1_, Kf_, M_, m_[0], m_[1],
+                           m_[2], gc, pc);
+    };
+    const bool hA = cc.phihA != 0.0;  // chain 2 (the A homopolymer's Q1) only when it is present
+    if (ck_ == 0) {
+        for (int s = 0; s < cc.N; ++s) term(0, s, s, cc.N - 1 - s);
+        for (int t = 0; t < cc.Nh; ++t) term(1, t, t, cc.Nh - 1 - t);
+        if (hA)
+            for (int t = 0; t < cc.Nh; ++t) term(2, t, t, cc.Nh - 1 - t);
+    } else {
+        // Terms s and L-1-s (L = N for chain 0, Nh for chains 1, 2) read the same two
+        // contour indices: window 0 holds s in [lo, hi), window 1 the mirrors (L-1-hi, L-1-lo].
+        const int w0 = ck_window(N_, ck_, 0), w1 = ck_window(N_, ck_, 1);
+        for (int kind = 0; kind < 2; ++kind) {
+            const int L = kind == 0 ? cc.N : cc.Nh;
+            for (int lo = 0; 2 * lo <= L - 1; lo += ck_) {
+                const int hi = std::min(lo + ck_, (L - 1) / 2 + 1);
+                PropArgs<T> aw = a;
+                window(aw, 0, lo, hi - 1, st);
+                window(aw, 1, L - hi, L - 1 - lo, st);
+                for (int s = lo; s < hi; ++s) {
+                    const int m = L - 1 - s, is = w0 + (s - lo), im = w1 + (m - (L - hi));
+                    for (int chain = kind == 0 ? 0 : 1; chain <= (kind == 0 ? 0 : (hA ? 2 : 1)); ++chain) {
+                        term(chain, s, is, im);
+                        if (m != s) term(chain, m, im, is);
+                    }
+                }
+            }
         }
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -638,7 +748,7 @@ hipError_t PropEngine<T>::contour_launch(PropArgs<T> a, int nb, const ChainConst
             g[i].Cf = Cf_ + off;
             g[i].e4A = e4A_ + off;
             g[i].e4B = e4B_ + off;
-            g[i].qs = qs_ + off * (size_t)(cc.N + 1);
+            g[i].qs = qs_ + off * (size_t)qpitch_;
             gs[i] = i == 0 ? st : sx_[i - 1];
             first += cnt;
         }
@@ -663,6 +773,7 @@ hipError_t PropEngine<T>::contour_launch(PropArgs<T> a, int nb, const ChainConst
 template <typename T>
 hipError_t PropEngine<T>::qq_invariant(int slot, double* qq, hipStream_t st)
 {
+    if (ck_) return hipErrorNotSupported;  // needs every contour index
     std::vector<double> buf((size_t)(N_ + 1) * nblk_);
     double* dtmp = nullptr;
     hipError_t e = hipMalloc((void**)&dtmp, sizeof(double) * buf.size());

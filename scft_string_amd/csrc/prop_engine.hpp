@@ -21,6 +21,9 @@ struct KernelSet {
     void (*begin2)(const PropArgs<T>&, hipStream_t);
     void (*step2)(const PropArgs<T>&, int s, hipStream_t);
     const char* name;
+    // start a recompute segment from the stored q at contour index c (PL_RESUME)
+    void (*resume)(const PropArgs<T>&, int c, hipStream_t);
+    void (*resume2)(const PropArgs<T>&, int c, hipStream_t);
 };
 
 // kernel set for an (m0, m1, m2) grid, or nullptr when no instantiation exists
@@ -38,8 +41,10 @@ public:
     PropEngine& operator=(const PropEngine&) = delete;
     ~PropEngine();
 
-    // Allocates buffers for nbmax slots. Returns "" on success, else an error message.
-    std::string init(const int m[3], const double D[3], int N, int nbmax);
+    // Allocates buffers for nbmax slots. ck = 0 keeps q at every contour index; ck = K > 0
+    // keeps every K-th (ck_index) and recomputes the rest in windows (run, box_gradient).
+    // Returns "" on success, else an error message.
+    std::string init(const int m[3], const double D[3], int N, int nbmax, int ck = 0);
     // Recomputes the kinetic factor tables for a new box D.
     hipError_t set_box(const double D[3]);
 
@@ -58,7 +63,8 @@ public:
     // dQ1'/dD_i (zero when cc.phihA == 0) (host array).
     // hipErrorNotSupported when an axis does not factor into 2, 3, 5 or exceeds 1024.
     hipError_t box_gradient(int nb, const ChainConst& cc, double* out, hipStream_t st);
-    const cpx<T>* qstore() const { return qs_; }
+    int checkpoint_interval() const { return ck_; }
+    int qstore_entries() const { return qpitch_; }
     long long M() const { return M_; }
     int N() const { return N_; }
     int nbmax() const { return nbmax_; }
@@ -84,12 +90,20 @@ private:
         hipError_t (*setup)();
         void (*begin)(const PropArgs<T>&, hipStream_t);
         void (*step)(const PropArgs<T>&, int s, hipStream_t);
+        void (*resume)(const PropArgs<T>&, int c, hipStream_t);
     };
     std::vector<Family> fams_;
     bool tune_pending_ = false, tuned_ = false;
     std::string tune_log_;
     hipError_t contour(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st);
     hipError_t contour_launch(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st);
+    // checkpointed store: recompute contour indices lo..hi into window w (entries
+    // ck_window(w) + s - lo), from the checkpoint at or below lo
+    void window(PropArgs<T> a, int w, int lo, int hi, hipStream_t st);
+    // q-store entry of contour index s outside the windows
+    int store_index(int s, const ChainConst& cc) const { return ck_ ? ck_index(s, cc.N, cc.Nh, ck_) : s; }
+    hipError_t windowed_density(const PropArgs<T>& a, int nb, const ChainConst& cc, double* dphi,
+                                long long phistride, const int* d_slot2rep, hipStream_t st);
     struct GraphEntry {
         int nb;
         hipStream_t st;
@@ -106,6 +120,9 @@ private:
     hipError_t (*setup_)() = nullptr;
     void (*begin_)(const PropArgs<T>&, hipStream_t) = nullptr;
     void (*step_)(const PropArgs<T>&, int s, hipStream_t) = nullptr;
+    void (*resume_)(const PropArgs<T>&, int c, hipStream_t) = nullptr;
+    int ck_ = 0;      // checkpoint interval (0: every contour index kept)
+    int qpitch_ = 0;  // q-store entries per slot
     const char* path_ = "";
     cpx<T>* Ch_ = nullptr;
     cpx<T>* Cf_ = nullptr;

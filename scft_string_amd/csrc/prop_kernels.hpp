@@ -76,10 +76,13 @@ __device__ __forceinline__ void load_twiddles(cpx<T>* tw1, cpx<T>* tw2, const Pr
     for (int i = tid; i < N2; i += nt) tw2[i] = a.tw2[i];
 }
 
-enum PlaneMode { PL_BEGIN = 0, PL_STEP = 1, PL_LAST = 2 };
+// PL_RESUME starts a segment of the contour loop from a stored q (q-store index a.qsrc) at
+// contour index s, as PL_BEGIN starts it from q = 1 at index 0.
+enum PlaneMode { PL_BEGIN = 0, PL_STEP = 1, PL_LAST = 2, PL_RESUME = 3 };
 
-// plane_end: completes step s (PL_STEP/PL_LAST) or starts the contour (PL_BEGIN), then
-// (unless PL_LAST) launches step s+1's two forward 2-D transforms.
+// plane_end: completes step s (PL_STEP/PL_LAST), starts the contour (PL_BEGIN) or resumes it
+// from a stored q at index s (PL_RESUME), stores q where the q store keeps that index, then
+// (unless PL_LAST) launches the next step's two forward 2-D transforms.
 // grid: (m0 planes, nb slots), block NT, dynamic LDS = plane + twiddles.
 template <typename T, int N1, int N2, int NT>
 __global__ void __launch_bounds__(NT) k_plane_end(PropArgs<T> a, int s, int mode)
@@ -113,6 +116,14 @@ __global__ void __launch_bounds__(NT) k_plane_end(PropArgs<T> a, int s, int mode
 #pragma unroll
         for (int i = 0; i < PE; ++i) q[i] = {T(1), T(1)};
         sout = 0;
+    } else if (mode == PL_RESUME) {
+        const cpx<T>* qi = a.qs + ((size_t)slot * (size_t)a.qpitch + (size_t)a.qsrc) * (size_t)a.M + (size_t)x * NE;
+#pragma unroll
+        for (int i = 0; i < PE; ++i) {
+            const int e = tid + i * NT;
+            if (NE % NT == 0 || e < NE) q[i] = qi[e];
+        }
+        sout = s;
     } else {
         cpx<T> h0[PE];
         plane_load<T, N1, N2, NT>(buf, a.Ch + fo, tid);
@@ -143,11 +154,14 @@ __global__ void __launch_bounds__(NT) k_plane_end(PropArgs<T> a, int s, int mode
         __syncthreads();
     }
 
-    cpx<T>* qo = a.qs + ((size_t)slot * (size_t)(a.N + 1) + (size_t)sout) * (size_t)a.M + (size_t)x * NE;
+    const int qix = q_index(a, sout);
+    if (qix >= 0) {
+        cpx<T>* qo = a.qs + ((size_t)slot * (size_t)a.qpitch + (size_t)qix) * (size_t)a.M + (size_t)x * NE;
 #pragma unroll
-    for (int i = 0; i < PE; ++i) {
-        const int e = tid + i * NT;
-        if (NE % NT == 0 || e < NE) store_q(&qo[e], q[i]);
+        for (int i = 0; i < PE; ++i) {
+            const int e = tid + i * NT;
+            if (NE % NT == 0 || e < NE) store_q(&qo[e], q[i]);
+        }
     }
     if (mode == PL_LAST) return;
 
@@ -364,13 +378,13 @@ __global__ void __launch_bounds__(RED_NT) k_factors(const double* __restrict__ W
 
 // partial sums of q1'[N], q2'[Nh] (density.h:377-381) and q1'[Nh] (Q1 of the A homopolymer)
 template <typename T>
-__global__ void __launch_bounds__(RED_NT) k_qsum(const cpx<T>* __restrict__ qs, long long M, int N, int Nh,
-                                                 double* __restrict__ part)
+__global__ void __launch_bounds__(RED_NT) k_qsum(const cpx<T>* __restrict__ qs, long long M, int qpitch, int iN,
+                                                 int iH, double* __restrict__ part)
 {
     __shared__ double sh[RED_NT];
     const int slot = blockIdx.y;
-    const cpx<T>* qN = qs + ((size_t)slot * (N + 1) + N) * (size_t)M;
-    const cpx<T>* qH = qs + ((size_t)slot * (N + 1) + Nh) * (size_t)M;
+    const cpx<T>* qN = qs + ((size_t)slot * qpitch + iN) * (size_t)M;  // contour index N
+    const cpx<T>* qH = qs + ((size_t)slot * qpitch + iH) * (size_t)M;  // contour index Nh
     double sc = 0, sh2 = 0, sh1 = 0;
     for (long long r = (long long)blockIdx.x * RED_NT + threadIdx.x; r < M; r += (long long)gridDim.x * RED_NT) {
         sc += (double)qN[r].x;
@@ -439,6 +453,32 @@ __device__ __forceinline__ cpx<T> load_q(const cpx<T>* p)
 }
 constexpr int DU = 4;  // slot pairs per batch in k_density
 
+// One point of the densities from the raw Simpson sums (density.h:423-430, GPU_kernels.h:
+// 114-139): phi rows (phiA, phiB, phih, phicB), and the phis sums (rC, rA, rB, rhA, rhB).
+__device__ __forceinline__ void density_point(double cA, double cB, double hA, double hB, double sC, double sH,
+                                              double sHA, double* ph, long long r, long long M, double (&acc)[5])
+{
+    const double rCA = cA * sC, rCB = cB * sC, rhB = hB * sH, rhA = hA * sHA;
+    const double rA = rhA + rCA, rB = rhB + rCB;
+    ph[r] = rA;
+    ph[r + M] = rB;
+    ph[r + 2 * M] = rhA + rhB;
+    ph[r + 3 * M] = rCB;
+    acc[0] += rCA + rCB;
+    acc[1] += rA;
+    acc[2] += rB;
+    acc[3] += rhA;
+    acc[4] += rhB;
+}
+
+// the workgroup's phis partials to o[0..4]
+__device__ __forceinline__ void density_partials(double (&acc)[5], double* sh, double* o)
+{
+    for (int c = 0; c < 5; ++c) acc[c] = block_sum(acc[c], sh);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 5; ++c) o[c] = acc[c];
+}
+
 // Weights of the copolymer term q1[i] q2[N-i] in the A-block and B-block sums
 // (density.h:413-422): ends 1, the junction n1 in both, interior wt_odd_first from each
 // block's start. A weight of 1 or 0 adds v or exactly nothing, as the branches did.
@@ -469,7 +509,7 @@ __global__ void __launch_bounds__(RED_NT) k_density(const cpx<T>* __restrict__ q
     const double sC = scal[(size_t)slot * SCAL_N + 1], sH = scal[(size_t)slot * SCAL_N + 2],
                  sHA = scal[(size_t)slot * SCAL_N + 8];
     double* ph = phi + (size_t)slot2rep[slot] * (size_t)phistride;
-    double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+    double acc[5] = {0, 0, 0, 0, 0};
     for (long long r = (long long)blockIdx.x * RED_NT + threadIdx.x; r < M; r += (long long)gridDim.x * RED_NT) {
         double cA = 0, cB = 0;
         // read in pairs (s, N-s), DU pairs per batch: every load of a batch is in flight
@@ -523,27 +563,84 @@ __global__ void __launch_bounds__(RED_NT) k_density(const cpx<T>* __restrict__ q
             hB += w * ((double)zs.y * (double)zt.y);
             hA += w * ((double)zs.x * (double)zt.x);
         }
-        const double rCA = cA * sC, rCB = cB * sC, rhB = hB * sH, rhA = hA * sHA;
-        const double rA = rhA + rCA, rB = rhB + rCB;
-        ph[r] = rA;
-        ph[r + M] = rB;
-        ph[r + 2 * M] = rhA + rhB;
-        ph[r + 3 * M] = rCB;
-        a0 += rCA + rCB;
-        a1 += rA;
-        a2 += rB;
-        a3 += rhA;
-        a4 += rhB;
+        density_point(cA, cB, hA, hB, sC, sH, sHA, ph, r, M, acc);
     }
-    a0 = block_sum(a0, sh);
-    a1 = block_sum(a1, sh);
-    a2 = block_sum(a2, sh);
-    a3 = block_sum(a3, sh);
-    a4 = block_sum(a4, sh);
-    if (threadIdx.x == 0) {
-        double* o = part + ((size_t)slot * gridDim.x + blockIdx.x) * 5;
-        o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3; o[4] = a4;
+    density_partials(acc, sh, part + ((size_t)slot * gridDim.x + blockIdx.x) * 5);
+}
+
+// Checkpointed q store: the Simpson sums of k_density over recompute windows. The phi rows
+// of a slot first hold the raw sums (cA, cB, hA, hB), zeroed by k_dens_zero. k_dens_win adds
+// the pairs whose lower index s is in [s0, s1): copolymer (kind 0) pairs (s, N - s) or
+// homopolymer (kind 1) pairs (s, Nh - s), with index s at q-store entry aBase + s - aLo and
+// its partner t at bBase + t - bLo. k_dens_finish adds the homopolymer end terms 2 q[Nh] and
+// turns the raw sums into phi and the phis partials as k_density does.
+__global__ void __launch_bounds__(RED_NT) k_dens_zero(long long M, double* __restrict__ phi, long long phistride,
+                                                      const int* __restrict__ slot2rep)
+{
+    double* ph = phi + (size_t)slot2rep[blockIdx.y] * (size_t)phistride;
+    for (long long r = (long long)blockIdx.x * RED_NT + threadIdx.x; r < 4 * M; r += (long long)gridDim.x * RED_NT)
+        ph[r] = 0.0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RED_NT) k_dens_win(const cpx<T>* __restrict__ qs, long long M, int qpitch,
+                                                     ChainConst cc, int kind, int s0, int s1, int aLo, int aBase,
+                                                     int bLo, int bBase, double* __restrict__ phi,
+                                                     long long phistride, const int* __restrict__ slot2rep)
+{
+    const int slot = blockIdx.y;
+    const int N = cc.N, n1 = cc.n1, L = kind == 0 ? cc.N : cc.Nh;
+    const cpx<T>* q = qs + (size_t)slot * (size_t)qpitch * (size_t)M;
+    double* ph = phi + (size_t)slot2rep[slot] * (size_t)phistride;
+    for (long long r = (long long)blockIdx.x * RED_NT + threadIdx.x; r < M; r += (long long)gridDim.x * RED_NT) {
+        double u0 = 0, u1 = 0;  // kind 0: cA, cB; kind 1: hA, hB
+        for (int s = s0; s < s1; ++s) {
+            const int t = L - s;
+            const cpx<T> zs = load_q(q + (size_t)(aBase + s - aLo) * M + r);
+            const cpx<T> zt = load_q(q + (size_t)(bBase + t - bLo) * M + r);
+            if (kind == 0) {
+                const double v = (double)zs.x * (double)zt.y;
+                const CoW w = co_weight(s, n1, N);
+                u0 += w.a * v;
+                u1 += w.b * v;
+                if (t != s) {
+                    const double v2 = (double)zt.x * (double)zs.y;
+                    const CoW w2 = co_weight(t, n1, N);
+                    u0 += w2.a * v2;
+                    u1 += w2.b * v2;
+                }
+            } else {
+                const double w = (t != s) ? wt_odd_first(s) + wt_odd_first(t) : wt_odd_first(s);
+                u0 += w * ((double)zs.x * (double)zt.x);
+                u1 += w * ((double)zs.y * (double)zt.y);
+            }
+        }
+        const long long o = kind == 0 ? 0 : 2 * M;
+        ph[r + o] += u0;
+        ph[r + o + M] += u1;
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RED_NT) k_dens_finish(const cpx<T>* __restrict__ qs, long long M, int qpitch,
+                                                        int iH, const double* __restrict__ scal,
+                                                        double* __restrict__ phi, long long phistride,
+                                                        const int* __restrict__ slot2rep, double* __restrict__ part)
+{
+    __shared__ double sh[RED_NT];
+    const int slot = blockIdx.y;
+    const cpx<T>* qh = qs + ((size_t)slot * (size_t)qpitch + (size_t)iH) * (size_t)M;
+    const double sC = scal[(size_t)slot * SCAL_N + 1], sH = scal[(size_t)slot * SCAL_N + 2],
+                 sHA = scal[(size_t)slot * SCAL_N + 8];
+    double* ph = phi + (size_t)slot2rep[slot] * (size_t)phistride;
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (long long r = (long long)blockIdx.x * RED_NT + threadIdx.x; r < M; r += (long long)gridDim.x * RED_NT) {
+        const cpx<T> zh = load_q(qh + r);
+        const double cA = ph[r], cB = ph[r + M];
+        const double hA = 2.0 * (double)zh.x + ph[r + 2 * M], hB = 2.0 * (double)zh.y + ph[r + 3 * M];
+        density_point(cA, cB, hA, hB, sC, sH, sHA, ph, r, M, acc);
+    }
+    density_partials(acc, sh, part + ((size_t)slot * gridDim.x + blockIdx.x) * 5);
 }
 
 __global__ void __launch_bounds__(RED_NT) k_phis(const double* __restrict__ part, int nblk, long long M,

@@ -309,7 +309,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT
     // load Cf after the Ch inverse and the factors after both inverses instead of spilling.
     constexpr bool EARLY = NE <= 4096;
     cpx<T> uh[P::IY2][P::B1], uf[P::IY2][P::B1];
-    if (mode != PL_BEGIN) {
+    if (mode != PL_BEGIN && mode != PL_RESUME) {
         P::inv_load(a.Ch + fo, uh, tid);
         if constexpr (EARLY) P::inv_load(a.Cf + fo, uf, tid);
     }
@@ -339,6 +339,16 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT
 #pragma unroll
             for (int a2 = 0; a2 < A2; ++a2) q[it][a2] = {T(1), T(1)};
         sout = 0;
+    } else if (mode == PL_RESUME) {
+        if constexpr (!EARLY) load_factors();
+        const cpx<T>* qi = a.qs + ((size_t)slot * (size_t)a.qpitch + (size_t)a.qsrc) * (size_t)a.M + (size_t)x * NE;
+#pragma unroll
+        for (int it = 0; it < IZ1; ++it) {
+            const int w = tid + it * NT;
+#pragma unroll
+            for (int a2 = 0; a2 < A2; ++a2) q[it][a2] = P::z1_valid(w) ? qi[P::z1_elem(w, a2)] : cpx<T>{T(0), T(0)};
+        }
+        sout = s;
     } else {
         cpx<T> h1[IZ1][A2];
         P::inv_regs(uh, buf, twz, twy, q, tid);  // q holds qs0 for now
@@ -358,13 +368,16 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT
             }
         sout = s + 1;
     }
-    cpx<T>* qo = a.qs + ((size_t)slot * (size_t)(a.N + 1) + (size_t)sout) * (size_t)a.M + (size_t)x * NE;
+    const int qix = q_index(a, sout);
+    if (qix >= 0) {
+        cpx<T>* qo = a.qs + ((size_t)slot * (size_t)a.qpitch + (size_t)qix) * (size_t)a.M + (size_t)x * NE;
 #pragma unroll
-    for (int it = 0; it < IZ1; ++it) {
-        const int w = tid + it * NT;
-        if (P::z1_valid(w))
+        for (int it = 0; it < IZ1; ++it) {
+            const int w = tid + it * NT;
+            if (P::z1_valid(w))
 #pragma unroll
-            for (int a2 = 0; a2 < A2; ++a2) store_q(&qo[P::z1_elem(w, a2)], q[it][a2]);
+                for (int a2 = 0; a2 < A2; ++a2) store_q(&qo[P::z1_elem(w, a2)], q[it][a2]);
+        }
     }
     if (mode == PL_LAST) return;
 

@@ -28,7 +28,7 @@ constexpr int PC_NT = 256;     // threads per workgroup of every pencil kernel
 constexpr int PC_EMAX = 2048;  // complex elements (lines x length) per workgroup transform
 constexpr int PC_LMAX = 1024;  // longest axis
 
-enum PencilMode { PZ_BEGIN = 0, PZ_STEP = 1, PZ_LAST = 2, PZ_MID = 3 };
+enum PencilMode { PZ_BEGIN = 0, PZ_STEP = 1, PZ_LAST = 2, PZ_MID = 3, PZ_RESUME = 4 };
 
 __host__ __device__ constexpr int pc_bpt(int R) { return (PC_EMAX / R + PC_NT - 1) / PC_NT; }
 __host__ __device__ constexpr int pc_pitch(int nlines) { return nlines + ((nlines & 1) ? 0 : 1); }
@@ -152,6 +152,8 @@ constexpr int PC_EPT = PC_EMAX / PC_NT;      // in k_py, k_px (lines x length <=
 //             then Ch <- Fz(E1/4 q), Cf <- Fz(E1/2 q) with step s+1's species
 //   PZ_LAST:  as PZ_STEP without the forward part
 //   PZ_BEGIN: q(0) = 1 stored, Ch <- Fz(E1/4), Cf <- Fz(E1/2) for step 0 (density.h:351-355)
+//   PZ_RESUME: as PZ_BEGIN from the stored q at q-store index a.qsrc, contour index s
+// q is stored only where the q store keeps its contour index (q_index).
 // LDS: lines [0, nlz) hold Ch, [nlz, 2 nlz) hold Cf, pitch pc_pitch(2 nlz), then m2 twiddles.
 template <typename T, int L = 0>
 __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, int nlz_arg)
@@ -160,6 +162,7 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
     const int m2 = L > 0 ? L : a.m2;
     const int nlz = L > 0 ? pc_ct_nlz(L) : nlz_arg;
     const int nfl = mode == PZ_MID ? 1 : 2;
+    const bool fresh = mode == PZ_BEGIN || mode == PZ_RESUME;  // q not computed from Ch, Cf
     const int nlines = nfl * nlz, P = pc_pitch(nlines);
     cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
     cpx<T>* tw = buf + (size_t)m2 * P;
@@ -178,12 +181,12 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
         const bool ok = e < ne;
         va[i] = ok ? a.e4A[fo + e] : T(0);
         vb[i] = ok ? a.e4B[fo + e] : T(0);
-        c0[i] = (ok && mode != PZ_BEGIN) ? a.Ch[fo + e] : cpx<T>{T(0), T(0)};
-        c1[i] = (ok && nfl == 2 && mode != PZ_BEGIN) ? a.Cf[fo + e] : cpx<T>{T(0), T(0)};
+        c0[i] = (ok && !fresh) ? a.Ch[fo + e] : cpx<T>{T(0), T(0)};
+        c1[i] = (ok && nfl == 2 && !fresh) ? a.Cf[fo + e] : cpx<T>{T(0), T(0)};
     }
     for (int i = tid; i < m2; i += PC_NT) tw[i] = a.tw2[i];
 
-    if (mode != PZ_BEGIN) {
+    if (!fresh) {
 #pragma unroll
         for (int i = 0; i < PZ_EPT; ++i) {
             const int e = tid + i * PC_NT;
@@ -222,8 +225,10 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
         return;
     }
 
-    const int sout = mode == PZ_BEGIN ? 0 : s + 1;
-    cpx<T>* qo = a.qs + ((size_t)slot * (size_t)(a.N + 1) + (size_t)sout) * (size_t)a.M + (size_t)l0 * m2;
+    const int sout = mode == PZ_BEGIN ? 0 : mode == PZ_RESUME ? s : s + 1;
+    const int qix = q_index(a, sout);
+    cpx<T>* qo = a.qs + ((size_t)slot * (size_t)a.qpitch + (size_t)(qix >= 0 ? qix : 0)) * (size_t)a.M + (size_t)l0 * m2;
+    const cpx<T>* qi = a.qs + ((size_t)slot * (size_t)a.qpitch + (size_t)a.qsrc) * (size_t)a.M + (size_t)l0 * m2;
     const bool qa = s < a.n1, pb = s < a.n2;        // this step's species
     const bool na = sout < a.n1, nbb = sout < a.n2;  // next step's species
 #pragma unroll
@@ -235,18 +240,20 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
                 cpx<T> q;
                 if (mode == PZ_BEGIN) {
                     q = {T(1), T(1)};
+                } else if (mode == PZ_RESUME) {
+                    q = qi[e];
                 } else {
                     const cpx<T> h0 = buf[z * P + l], h1 = buf[z * P + nlz + l];
                     q.x = Boltz<T>::rich(h0.x, h1.x, qa ? va[i] : vb[i]);
                     q.y = Boltz<T>::rich(h0.y, h1.y, pb ? vb[i] : va[i]);
                 }
-                store_q(&qo[e], q);
+                if (qix >= 0) store_q(&qo[e], q);
                 if (mode != PZ_LAST) {
                     const T ex = na ? va[i] : vb[i], ey = nbb ? vb[i] : va[i];
                     buf[z * P + l] = {Boltz<T>::mul4(q.x, ex), Boltz<T>::mul4(q.y, ey)};
                     buf[z * P + nlz + l] = {Boltz<T>::mul2(q.x, ex), Boltz<T>::mul2(q.y, ey)};
                 }
-            } else if (mode == PZ_BEGIN) {
+            } else if (fresh) {
                 buf[z * P + l] = {T(0), T(0)};
                 buf[z * P + nlz + l] = {T(0), T(0)};
             }

@@ -13,11 +13,15 @@ struct PropArgs {
     int N, n1, n2, Nh;     // contour steps, block split, homopolymer length
     int nb;                // active batch slots
     int pencil_ct;         // line kernels: use the compile-time-length ones where they exist
-    cpx<T>* Ch;           // [slot][M] half-step sandwich work field
+    cpx<T>* Ch;            // [slot][M] half-step sandwich work field
     cpx<T>* Cf;            // [slot][M] full-step sandwich work field
     const T* e4A;          // [slot][M] quarter-step factor of species A (Boltz<T> form)
     const T* e4B;          // [slot][M] the same for species B
-    cpx<T>* qs;            // [slot][N+1][M] stored (q1, q2)
+    cpx<T>* qs;            // [slot][qpitch][M] stored (q1, q2)
+    int qpitch;            // q-store entries per slot (N + 1 when every contour index is kept)
+    int qck;               // 0: every contour index is kept; K > 0: checkpoints (ck_index)
+    int qlo, qhi, qbase;   // qck > 0 and qlo <= qhi: a recompute window, see q_index
+    int qsrc;              // q-store index a resumed segment starts from
     const cpx<T>* tw0;     // twiddle tables, lengths m0, m1, m2
     const cpx<T>* tw1;
     const cpx<T>* tw2;
@@ -36,6 +40,27 @@ __device__ __forceinline__ void store_q(cpx<T>* p, cpx<T> v)
     using V = T __attribute__((ext_vector_type(2)));
     const V w = {v.x, v.y};
     __builtin_nontemporal_store(w, reinterpret_cast<V*>(p));
+}
+
+// Checkpointed q store (qck = K > 0), per slot: entries 0 .. N/K hold contour indices
+// 0, K, 2K, ...; entry N/K + 1 holds index N and N/K + 2 index Nh (the partition
+// functions read them); then two recompute windows of K + 1 entries each.
+__host__ __device__ constexpr int ck_pitch(int N, int K) { return N / K + 3 + 2 * (K + 1); }
+__host__ __device__ constexpr int ck_window(int N, int K, int w) { return N / K + 3 + w * (K + 1); }
+__host__ __device__ constexpr int ck_index(int s, int N, int Nh, int K)
+{
+    return s % K == 0 ? s / K : s == N ? N / K + 1 : s == Nh ? N / K + 2 : -1;
+}
+
+// q-store index of contour index s, -1 when the store does not keep it: s itself (full
+// store), a checkpoint entry (the forward pass of a checkpointed store), or, in a recompute
+// window, qbase + s - qlo for qlo <= s <= qhi.
+template <typename T>
+__device__ __forceinline__ int q_index(const PropArgs<T>& a, int s)
+{
+    if (a.qck == 0) return s;
+    if (a.qlo <= a.qhi) return (s >= a.qlo && s <= a.qhi) ? a.qbase + (s - a.qlo) : -1;
+    return ck_index(s, a.N, a.Nh, a.qck);
 }
 
 // Per-slot scalars (density.h:376-399) in FP64 from the factored sums:

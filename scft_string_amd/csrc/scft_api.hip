@@ -371,16 +371,17 @@ scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
     if (p->phihA < 0 || (p->phihA > 0 && c->cc.Nh > c->cc.n1)) return SCFT_ERR_INVALID;
     if (p->ens == 1 && p->phidb + p->phihA > 1.0) return SCFT_ERR_INVALID;
     if (p->anderson < 0 || p->anderson > 40 || (p->anderson > 0 && p->dostring != 0)) return SCFT_ERR_INVALID;
+    if (p->ckpt < 0) return SCFT_ERR_INVALID;
 
     if (hipSetDevice(p->device) != hipSuccess) return SCFT_ERR_HIP;
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return SCFT_ERR_HIP;
     std::string msg;
     if (p->precision == SCFT_FP32) {
         c->pe32 = std::make_unique<PropEngine<float>>();
-        msg = c->pe32->init(p->m, p->D, p->N, c->R);
+        msg = c->pe32->init(p->m, p->D, p->N, c->R, p->ckpt);
     } else {
         c->pe64 = std::make_unique<PropEngine<double>>();
-        msg = c->pe64->init(p->m, p->D, p->N, c->R);
+        msg = c->pe64->init(p->m, p->D, p->N, c->R, p->ckpt);
     }
     if (msg.empty() && p->autotune) {
         if (c->pe32) c->pe32->request_autotune();
@@ -740,7 +741,7 @@ scft_status scft_free_energy(scft_ctx* c, double* FE, double* FEs, double* FEs0,
     const bool fe64 = c->pe32 && c->p.fe_fp64;
     if (fe64 && !c->pe64_fe) {
         c->pe64_fe = std::make_unique<PropEngine<double>>();
-        const std::string msg = c->pe64_fe->init(c->p.m, c->p.D, c->p.N, c->R);
+        const std::string msg = c->pe64_fe->init(c->p.m, c->p.D, c->p.N, c->R, c->p.ckpt);
         if (!msg.empty()) {
             c->pe64_fe.reset();
             return fail(c, SCFT_ERR_NOMEM, "FP64 free-energy engine: " + msg);
