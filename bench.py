@@ -197,32 +197,45 @@ def batched_fp64_rate(W0, warmup, steps):
 def adjoint_report():
     """Box derivative d lnQ/dD (adjoint of props) at 64^3, N = 100, chiN = 20, f = 0.5, FP32,
     on a 1-replica and an 8-replica string (lamellar -> cylinder interpolants, seed 2024),
-    with D swept over 5.6 ... 6.4. Reports the mean time of props alone and props plus the
-    gradient (host-buffer API, so both include the PCIe copies of W and phi)."""
+    with D swept over 5.6 ... 6.4, with the full q store and with checkpoints every 10
+    contour steps. Reports the mean time of props alone and props plus the gradient
+    (host-buffer API, so both include the PCIe copies of W and phi), the q-store size, and
+    the largest difference of the checkpointed gradient from the full-store one."""
     from scft_string_amd import Context
 
     m, N = (64, 64, 64), 100
+    M = m[0] * m[1] * m[2]
     out = {"mode": "adjoint", "workload": "box gradient d lnQ/dD, 64^3, N=100, chiN=20, f=0.5, FP32",
            "D_sweep": [5.6 + 0.1 * i for i in range(9)]}
     for R in (1, 8):
         W = initial_fields(m, (6.0,) * 3, max(R, 2))[:R]
-        tp = tg = 0.0
-        dmax = 0.0
-        with Context(m, (5.6,) * 3, N, 0.5, 0.1, 1, 0.8, 20.0, nrep=R, precision="fp32", dostring=0) as ctx:
-            ctx.props_batch(W)
-            ctx.dlnq_dbox(W)
-            for D in out["D_sweep"]:
-                ctx.set_box((D, D, D))
-                t = time.perf_counter()
+        grads = {}
+        for ck in (0, 10):
+            tp = tg = 0.0
+            dmax = 0.0
+            grads[ck] = []
+            with Context(m, (5.6,) * 3, N, 0.5, 0.1, 1, 0.8, 20.0, nrep=R, precision="fp32", dostring=0,
+                         ckpt=ck) as ctx:
                 ctx.props_batch(W)
-                tp += time.perf_counter() - t
-                t = time.perf_counter()
-                _, d = ctx.dlnq_dbox(W)
-                tg += time.perf_counter() - t
-                dmax = max(dmax, float(np.max(np.abs(d))))
-        n = len(out["D_sweep"])
-        out[f"replicas_{R}"] = {"props_ms": round(tp / n * 1e3, 2), "props_plus_gradient_ms": round(tg / n * 1e3, 2),
-                                "gradient_over_props": round((tg - tp) / tp, 3), "max_abs_dlnQ_dD": dmax}
+                ctx.dlnq_dbox(W)
+                for D in out["D_sweep"]:
+                    ctx.set_box((D, D, D))
+                    t = time.perf_counter()
+                    ctx.props_batch(W)
+                    tp += time.perf_counter() - t
+                    t = time.perf_counter()
+                    _, d = ctx.dlnq_dbox(W)
+                    tg += time.perf_counter() - t
+                    dmax = max(dmax, float(np.max(np.abs(d))))
+                    grads[ck].append(d)
+            n = len(out["D_sweep"])
+            entries = N + 1 if ck == 0 else N // ck + 3 + 2 * (ck + 1)
+            key = f"replicas_{R}" + ("" if ck == 0 else f"_ckpt{ck}")
+            out[key] = {"props_ms": round(tp / n * 1e3, 2), "props_plus_gradient_ms": round(tg / n * 1e3, 2),
+                        "gradient_over_props": round((tg - tp) / tp, 3), "max_abs_dlnQ_dD": dmax,
+                        "q_store_GB": round(entries * M * 8 * R / 1e9, 3)}
+        g0, g1 = np.array(grads[0]), np.array(grads[10])
+        out[f"replicas_{R}_ckpt10"]["max_rel_diff_vs_full_store"] = float(np.max(np.abs(g1 - g0)) / np.max(np.abs(g0)))
     print(json.dumps(out), flush=True)
 
 

@@ -358,8 +358,13 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
     }
     if (e != hipSuccess) return std::string("hipFuncSetAttribute: ") + hipGetErrorString(e);
     nblk_ = (int)std::min<long long>(1024, (M_ + RED_NT - 1) / RED_NT);
+    // A failed hipMalloc also becomes the thread's last error, which a later
+    // hipGetLastError() after kernel launches would report: it is read off here.
     auto alloc = [&](void** p, size_t bytes) -> bool {
-        if (hipMalloc(p, bytes) != hipSuccess) return false;
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
         bytes_ += bytes;
         return true;
     };
@@ -624,8 +629,10 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
         const size_t fld = sizeof(cpx<T>) * (size_t)M_ * nbmax_;
         const size_t pb = sizeof(double) * (size_t)nbmax_ * 3 * nblk_ * 3;
         if ((e = hipMalloc((void**)&K0_, fld)) != hipSuccess || (e = hipMalloc((void**)&K1_, fld)) != hipSuccess ||
-            (e = hipMalloc((void**)&Kf_, fld)) != hipSuccess || (e = hipMalloc((void**)&gpart_, pb)) != hipSuccess)
+            (e = hipMalloc((void**)&Kf_, fld)) != hipSuccess || (e = hipMalloc((void**)&gpart_, pb)) != hipSuccess) {
+            (void)hipGetLastError();  // not left for the next launch check to report
             return e;
+        }
         bytes_ += 3 * fld + pb;
         const int lds = (int)pencil_lds_max<T>();
         const void* fns[] = {(const void*)k_gz<T>, (const void*)k_gx<T>, (const void*)k_py<T, false>,
@@ -777,7 +784,10 @@ hipError_t PropEngine<T>::qq_invariant(int slot, double* qq, hipStream_t st)
     std::vector<double> buf((size_t)(N_ + 1) * nblk_);
     double* dtmp = nullptr;
     hipError_t e = hipMalloc((void**)&dtmp, sizeof(double) * buf.size());
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return e;
+    }
     hipLaunchKernelGGL((k_qq<T>), dim3(nblk_, N_ + 1), dim3(RED_NT), 0, st, qs_, M_, N_, slot, dtmp);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(buf.data(), dtmp, sizeof(double) * buf.size(), hipMemcpyDeviceToHost, st);

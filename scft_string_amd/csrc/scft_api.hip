@@ -391,6 +391,7 @@ scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
         fprintf(stderr, "scft_ctx_create: %s\n", msg.c_str());
         (void)hipStreamDestroy(c->st);
         c->st = nullptr;
+        (void)hipGetLastError();  // a failed allocation must not surface in a later context's launch check
         return msg.find("no ") == 0 ? SCFT_ERR_UNSUPPORTED : SCFT_ERR_NOMEM;
     }
     const long long M = c->M;
@@ -429,6 +430,7 @@ scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
         for (auto& a : c->and_) a.G.assign(L * L, 0.0);
     }
     if (!ok) {
+        (void)hipGetLastError();
         scft_ctx_destroy(c.release());
         return SCFT_ERR_NOMEM;
     }
@@ -744,6 +746,7 @@ scft_status scft_free_energy(scft_ctx* c, double* FE, double* FEs, double* FEs0,
         const std::string msg = c->pe64_fe->init(c->p.m, c->p.D, c->p.N, c->R, c->p.ckpt);
         if (!msg.empty()) {
             c->pe64_fe.reset();
+            (void)hipGetLastError();
             return fail(c, SCFT_ERR_NOMEM, "FP64 free-energy engine: " + msg);
         }
     }

@@ -111,3 +111,15 @@ def test_checkpointed_string_iterations_match_full_store():
 def test_invalid_checkpoint_interval_rejected():
     with pytest.raises(Exception):
         Context((16, 16, 16), (3.0, 3.0, 3.0), 20, 0.5, 0.1, 1, 0.8, 15.0, ckpt=-1)
+
+
+def test_failed_allocation_does_not_poison_the_next_context():
+    """A context whose q store cannot be allocated (512^3 FP32, N = 2000: 2 TB) fails with
+    an error, and a context created after it runs props normally: the failed hipMalloc is
+    not left behind as the error a later launch check reports."""
+    with pytest.raises(Exception):
+        Context((512, 512, 512), (8.0, 8.0, 8.0), 2000, 0.5, 0.1, 1, 0.8, 15.0, precision="fp32")
+    m, D = (16, 16, 16), (3.0, 3.0, 3.0)
+    with Context(m, D, 20, 0.5, 0.1, 1, 0.8, 15.0, precision="fp64", ckpt=4) as ctx:
+        _, lnQ, _ = ctx.props_batch(_fields(m, D, 1, seed=1))
+    assert np.isfinite(lnQ[0])
