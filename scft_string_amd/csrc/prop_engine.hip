@@ -20,7 +20,10 @@ template <typename T, int N0, int N1, int N2>
 struct KS {
     static constexpr int NE = N1 * N2;
     static constexpr int NTP = NE <= 256 ? 64 : NE <= 1024 ? 128 : NE <= 4096 ? 256 : NE <= 8192 ? 512 : 1024;
-    static constexpr int TC = 64;
+    // (y, z) columns per workgroup of the column pass. Measured on MI355X per contour step:
+    // 32 for x-length 48 and 96 (96^3 FP64 0.977 -> 0.800 ms, FP32 -5%, 48^3 -3%; 16 is no
+    // better), 64 elsewhere (64^3 is 1.5-2.5% slower with 32, 128^3 unchanged).
+    static constexpr int TC = (N0 == 48 || N0 == 96) ? 32 : 64;
     static constexpr int NTC = 256;
     static constexpr size_t lds_plane = (size_t)(N1 * PlaneGeom<N1, N2>::P2 + N1 + N2) * sizeof(cpx<T>);
     static constexpr size_t lds_col = (size_t)(N0 * TC + N0) * sizeof(cpx<T>) + (size_t)N0 * sizeof(T);
@@ -115,7 +118,8 @@ struct KS {
     static KernelSet<T> make()
     {
         KernelSet<T> k{N0,      N1,      N2,      &setup_legacy, &begin_legacy,  &step_legacy,
-                       nullptr, nullptr, nullptr, "stockham",    &resume_legacy, nullptr};
+                       nullptr, nullptr, nullptr, "stockham",    &resume_legacy, nullptr,
+                       lds_plane > 80 * 1024};
         if (two_ok) {
             k.setup2 = &setup_two;
             k.begin2 = &begin_two;
@@ -334,10 +338,16 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
     const char* pen_rt = getenv("SCFT_PENCIL_RT");  // =1: runtime-length line kernels only
     pencil_ct_ = !(pen_rt && pen_rt[0] == '1');
     ks_ = (pen && pen[0] == '1') ? nullptr : find_kernels<T>(m[0], m[1], m[2]);
+    const char* legacy = getenv("SCFT_LEGACY_FFT");
+    const bool force_legacy = legacy && legacy[0] == '1';
+    // Without a two-stage set, a Stockham plane that fills a CU's LDS runs one workgroup
+    // per CU; the line kernels are faster there (96^3 FP64: 0.529 against 0.805 ms per
+    // contour step for 8 replicas).
+    const KernelSet<T>* kfound = ks_;
+    if (ks_ && !ks_->step2 && ks_->big_plane && !force_legacy && pencil_supported(m[0], m[1], m[2])) ks_ = nullptr;
     hipError_t e = hipSuccess;
     if (ks_) {
-        const char* legacy = getenv("SCFT_LEGACY_FFT");
-        const bool use_two = ks_->step2 && !(legacy && legacy[0] == '1');
+        const bool use_two = ks_->step2 && !force_legacy;
         setup_ = use_two ? ks_->setup2 : ks_->setup;
         begin_ = use_two ? ks_->begin2 : ks_->begin;
         step_ = use_two ? ks_->step2 : ks_->step;
@@ -398,9 +408,10 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
             return "stream / event creation failed";
     // kernel families this grid can run, for the autotuner
     fams_.clear();
-    if (ks_ && ks_->step2) fams_.push_back({"two-stage", ks_->setup2, ks_->begin2, ks_->step2, ks_->resume2});
-    if (ks_ && ks_->step != ks_->step2)
-        fams_.push_back({"stockham", ks_->setup, ks_->begin, ks_->step, ks_->resume});
+    if (kfound && kfound->step2)
+        fams_.push_back({"two-stage", kfound->setup2, kfound->begin2, kfound->step2, kfound->resume2});
+    if (kfound && kfound->step != kfound->step2)
+        fams_.push_back({"stockham", kfound->setup, kfound->begin, kfound->step, kfound->resume});
     if (pencil_supported(m[0], m[1], m[2]))
         fams_.push_back({"pencil", &pencil_setup<T>, &pencil_begin<T>, &pencil_step<T>, &pencil_resume<T>});
     return "";

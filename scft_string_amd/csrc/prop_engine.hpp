@@ -24,6 +24,8 @@ struct KernelSet {
     // start a recompute segment from the stored q at contour index c (PL_RESUME)
     void (*resume)(const PropArgs<T>&, int c, hipStream_t);
     void (*resume2)(const PropArgs<T>&, int c, hipStream_t);
+    // the LDS Stockham plane needs more than half a CU's LDS (one workgroup per CU)
+    bool big_plane;
 };
 
 // kernel set for an (m0, m1, m2) grid, or nullptr when no instantiation exists

@@ -97,10 +97,10 @@ __device__ void pc_fft(cpx<T>* buf, int L, int nl, int P, const cpx<T>* tw, int 
 // 128^3, ...). Their tiles are fixed: nlz = PC_EMAX / 2 / L z lines, t = PC_EMAX / L z
 // columns, and the transform is the compile-time Stockham of fft_engine.hpp (same radix
 // plan and butterflies as pc_fft). Other lengths take the runtime kernels (L = 0).
-#define PC_CT_LENGTHS(X) X(128) X(150) X(160) X(192) X(200) X(256)
+#define PC_CT_LENGTHS(X) X(96) X(128) X(150) X(160) X(192) X(200) X(256)
 __host__ __device__ constexpr bool pc_ct_len(int L)
 {
-    return L == 128 || L == 150 || L == 160 || L == 192 || L == 200 || L == 256;
+    return L == 96 || L == 128 || L == 150 || L == 160 || L == 192 || L == 200 || L == 256;
 }
 __host__ __device__ constexpr int pc_ct_nlz(int L) { return PC_EMAX / 2 / L > 0 ? PC_EMAX / 2 / L : 1; }
 __host__ __device__ constexpr int pc_ct_t(int L) { return PC_EMAX / L > 0 ? PC_EMAX / L : 1; }

@@ -192,6 +192,21 @@ def test_props_large_grids_vs_oracle(case):
     assert dl <= tol and dphi <= tol
 
 
+def test_96cube_fp64_stockham_and_line_kernels_agree(monkeypatch):
+    """96^3 FP64 has no two-stage set, and its Stockham plane fills a CU's LDS, so it runs on
+    the line kernels by default; SCFT_LEGACY_FFT=1 keeps the Stockham kernels. Both agree."""
+    m, D = (96, 96, 96), (8.0, 8.0, 8.0)
+    W = _fields(m, D, seed=21, amp=1.5)[None]
+    out = {}
+    for leg in ("0", "1"):
+        monkeypatch.setenv("SCFT_LEGACY_FFT", leg)
+        with Context(m, D, 8, 0.5, 0.125, 2, 0.7, chi=15.0, precision="fp64") as ctx:
+            assert ctx.fft_path == ("stockham" if leg == "1" else "pencil")
+            out[leg] = ctx.props_batch(W)
+    assert abs(out["0"][1][0] - out["1"][1][0]) <= 1e-12 * max(1.0, abs(out["1"][1][0]))
+    assert _rel(out["0"][0], out["1"][0]) <= 1e-12
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_two_stage_and_stockham_paths_agree(prec, monkeypatch):
     """The two-stage register FFT and the LDS Stockham kernels compute the same props."""

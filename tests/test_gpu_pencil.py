@@ -128,10 +128,10 @@ def test_pencil_and_two_stage_agree(prec, tol, monkeypatch):
     assert dl <= tol and dphi <= 10 * tol
 
 
-@pytest.mark.parametrize("m", [(128, 150, 160), (150, 12, 200)])
+@pytest.mark.parametrize("m", [(128, 150, 160), (150, 12, 200), (96, 20, 96)])
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
 def test_compile_time_line_kernels_match_runtime(m, prec, tol, monkeypatch):
-    """Axes of lengths 128, 150, 160, 192, 200, 256 run line kernels compiled for that length
+    """Axes of lengths 96, 128, 150, 160, 192, 200, 256 run line kernels compiled for that length
     (fixed tiles, unrolled passes); SCFT_PENCIL_RT=1 forces the runtime-length plan. Both
     compute the same transform: props and the box gradient must agree."""
     D = tuple(0.08 * v for v in m)
