@@ -49,7 +49,8 @@ def initial_fields(m, D, R, seed=2024):
 
 def cpu_baseline(m, D, N, R_active, props_per_iter, budget_s=20.0):
     """Times the FP64 oracle (C, OpenMP) on a bounded sample: one props of one replica at the
-    full grid and contour length; scaled to string iterations/s of R_active * props_per_iter."""
+    full grid and contour length; scaled to string iterations/s of R_active * props_per_iter.
+    Also one props on a single thread, reported beside it."""
     from oracle import orc
 
     threads = min(os.cpu_count() or 1, 16)
@@ -64,10 +65,17 @@ def cpu_baseline(m, D, N, R_active, props_per_iter, budget_s=20.0):
         if time.perf_counter() - t0 > budget_s * 0.5 or n >= 40:
             break
     dt = (time.perf_counter() - t0) / n
+    orc.set_threads(1)
+    t1 = time.perf_counter()
+    g.props(W, N=N, alpha=CFG["alpha"], phic=CFG["phidb"], f=CFG["f"], ens=CFG["ens"])
+    dt1 = time.perf_counter() - t1
+    orc.set_threads(threads)
     return {"value": 1.0 / (dt * R_active * props_per_iter), "unit": "string iters/s", "cores": threads,
             "kind": "port",
             "sample": f"{n} FP64 props of one replica at {m[0]}^3, N={N} ({dt:.2f} s each), scaled to "
-                      f"{R_active} replicas x {props_per_iter} props per iteration"}
+                      f"{R_active} replicas x {props_per_iter} props per iteration",
+            "single_thread": {"value": 1.0 / (dt1 * R_active * props_per_iter), "cores": 1,
+                              "sample": f"1 FP64 props ({dt1:.2f} s)"}}
 
 
 def free_energies(W, phi, lnQ, chi):

@@ -288,6 +288,22 @@ PropEngine<T>::~PropEngine()
     }
 }
 
+// Extra streams (default priority) and join events for n replica-group streams, created when
+// first needed. Streams take hardware queues in creation order, so an engine creates no
+// more than it runs: with several contexts on one GPU, unused streams would put every
+// context's working streams on the same few queues and serialise them.
+template <typename T>
+hipError_t PropEngine<T>::ensure_streams(int n)
+{
+    for (int i = 0; i + 1 < n && i + 1 < kMaxStreams; ++i) {
+        hipError_t e = hipSuccess;
+        if (!sx_[i]) e = hipStreamCreateWithFlags(&sx_[i], hipStreamNonBlocking);
+        if (e == hipSuccess && !join_[i]) e = hipEventCreateWithFlags(&join_[i], hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 template <typename T>
 hipError_t PropEngine<T>::enable_timing(bool on)
 {
@@ -399,13 +415,9 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
     use_graph_ = gr && gr[0] == '1';
     const char* ns = getenv("SCFT_STREAMS");
     nstreams_ = std::max(1, std::min(kMaxStreams, ns ? atoi(ns) : kDefaultStreams));
-    // extra streams at the default priority, plus fork/join events (no timing); all of them,
-    // so that the autotuner can pick any replica-group count
+    // fork/join events (no timing) and the extra streams this replica-group count uses
     if (hipEventCreateWithFlags(&fork_, hipEventDisableTiming) != hipSuccess) return "event creation failed";
-    for (int i = 0; i + 1 < kMaxStreams; ++i)
-        if (hipStreamCreateWithFlags(&sx_[i], hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&join_[i], hipEventDisableTiming) != hipSuccess)
-            return "stream / event creation failed";
+    if (ensure_streams(nstreams_) != hipSuccess) return "stream / event creation failed";
     // kernel families this grid can run, for the autotuner
     fams_.clear();
     if (kfound && kfound->step2)
@@ -438,6 +450,7 @@ hipError_t PropEngine<T>::tune(const PropArgs<T>& a, int nb, const ChainConst& c
         if ((e = fams_[f].setup()) != hipSuccess) break;
         for (int ns = 1; ns <= std::min(3, kMaxStreams) && e == hipSuccess; ++ns) {
             if (ns > 1 && ns > nb) break;
+            if ((e = ensure_streams(ns)) != hipSuccess) break;
             for (int gph = 0; gph < 2 && e == hipSuccess; ++gph) {
                 begin_ = fams_[f].begin;
                 step_ = fams_[f].step;

@@ -97,6 +97,7 @@ private:
     std::vector<Family> fams_;
     bool tune_pending_ = false, tuned_ = false;
     std::string tune_log_;
+    hipError_t ensure_streams(int n);
     hipError_t contour(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st);
     hipError_t contour_launch(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st);
     // checkpointed store: recompute contour indices lo..hi into window w (entries

@@ -5,8 +5,8 @@ The string run repeats main()'s order (scft.cu:1102-1155):
 - maxiter iterations of solve_field, with FreeE at k = 0 and k = maxiter - 1;
 - a final FreeE.
 The oracle runs the same sequence from the same win files, and the FEs file (10 decimals,
-scft.cu:843-850) must agree to its printed precision. The relaxation run checks the
-`relaxbox` key.
+scft.cu:843-850) must agree to its printed precision, with the full q store and with
+`ckpt = 5`. The relaxation run checks the `relaxbox` key.
 """
 import os
 import re
@@ -50,7 +50,8 @@ def _run(d, timeout=240):
     return out.stdout
 
 
-def test_string_run_matches_oracle(tmp_path):
+@pytest.mark.parametrize("ckpt", [0, 5])
+def test_string_run_matches_oracle(tmp_path, ckpt):
     m, D, N, R = (16, 16, 16), (4.0, 4.0, 4.0), 20, 4
     chi, f, alpha, phidb, ens, maxiter, prerelax = 15.0, 0.5, 0.1, 0.8, 1, 3, 100
     M = int(np.prod(m))
@@ -67,8 +68,8 @@ def test_string_run_matches_oracle(tmp_path):
     # the fields exactly as the driver parses them (6 decimals)
     W0 = np.stack([np.loadtxt(os.path.join(tmp_path, f"win{i}")).T.copy() for i in range(R)])
     _write_input(tmp_path, chi=chi, f=f, ens=ens, phidb=phidb, N=N, m=m, D=D, fix=(1, 1), readwin=1,
-                 justFE=0, dostring=1, strn=R, maxiter=maxiter, prerelax=prerelax, precision=64)
-    _say("driver: string run")
+                 justFE=0, dostring=1, strn=R, maxiter=maxiter, prerelax=prerelax, precision=64, ckpt=ckpt)
+    _say(f"driver: string run, ckpt = {ckpt}")
     log = _run(tmp_path)
     for name in ["FEs", "check"] + [f"win{i}" for i in range(R)] + [f"rhoA_{i}.vtk" for i in range(R)]:
         assert os.path.exists(os.path.join(tmp_path, name)), (name, log[-1000:])
