@@ -149,7 +149,8 @@ class Context:
 
     @property
     def fft_path(self):
-        """"two-stage", "stockham" or "pencil": the propagator's FFT kernels for this grid."""
+        """"two-stage" (or "two-stage-c32" / "two-stage-c64" at the other column tile width),
+        "stockham" or "pencil": the propagator's FFT kernels for this grid."""
         return lib().scft_fft_path(self._h).decode()
 
     @property

@@ -24,6 +24,8 @@ struct KS {
     // 32 for x-length 48 and 96 (96^3 FP64 0.977 -> 0.800 ms, FP32 -5%, 48^3 -3%; 16 is no
     // better), 64 elsewhere (64^3 is 1.5-2.5% slower with 32, 128^3 unchanged).
     static constexpr int TC = (N0 == 48 || N0 == 96) ? 32 : 64;
+    // the other width, for the two-stage column kernel: the autotuner may pick it
+    static constexpr int TC_ALT = TC == 64 ? 32 : 64;
     static constexpr int NTC = 256;
     static constexpr size_t lds_plane = (size_t)(N1 * PlaneGeom<N1, N2>::P2 + N1 + N2) * sizeof(cpx<T>);
     static constexpr size_t lds_col = (size_t)(N0 * TC + N0) * sizeof(cpx<T>) + (size_t)N0 * sizeof(T);
@@ -34,6 +36,10 @@ struct KS {
     static constexpr size_t lds2_plane = (size_t)(PG::LDSN + N1 + N2) * sizeof(cpx<T>);
     static constexpr size_t lds2_col = (size_t)(CG::LDSN + N0) * sizeof(cpx<T>) + (size_t)N0 * sizeof(T);
     static constexpr bool two_ok = PG::ok && CG::ok && lds2_plane <= 160 * 1024 && lds2_col <= 160 * 1024;
+    using CGA = C2Geom<N0, TC_ALT>;
+    static constexpr size_t lds2_col_alt = (size_t)(CGA::LDSN + N0) * sizeof(cpx<T>) + (size_t)N0 * sizeof(T);
+    static constexpr bool alt_ok = two_ok && CGA::ok && lds2_col_alt <= 160 * 1024;
+    static constexpr int ntile_alt = (N1 * N2 + TC_ALT - 1) / TC_ALT;
     static_assert(legacy_ok || two_ok, "no kernel fits in LDS for this grid");
     static_assert(fft_len_supported(N0) && fft_len_supported(N1) && fft_len_supported(N2), "radix");
 
@@ -86,10 +92,28 @@ struct KS {
             e = hipFuncSetAttribute((const void*)k_plane_mid2<T, N1, N2>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds2_plane);
             if (e != hipSuccess) return e;
+            if constexpr (alt_ok) {
+                e = hipFuncSetAttribute((const void*)k_column2<T, N0, N1, N2, TC_ALT>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2_col_alt);
+                if (e != hipSuccess) return e;
+            }
             return hipFuncSetAttribute((const void*)k_column2<T, N0, N1, N2, TC>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2_col);
         }
         return hipErrorNotSupported;
+    }
+    // column pass over rows fields (Ch of every slot, then nf Cf), at the tile width a.coltile
+    // asks for (0: TC)
+    static void column_two(const PropArgs<T>& a, int rows, int nf, hipStream_t st)
+    {
+        if constexpr (alt_ok) {
+            if (a.coltile == TC_ALT) {
+                hipLaunchKernelGGL((k_column2<T, N0, N1, N2, TC_ALT>), dim3(ntile_alt, rows), dim3(CGA::NT),
+                                   lds2_col_alt, st, a, a.nb, nf);
+                return;
+            }
+        }
+        hipLaunchKernelGGL((k_column2<T, N0, N1, N2, TC>), dim3(ntile, rows), dim3(CG::NT), lds2_col, st, a, a.nb, nf);
     }
     static void begin_two(const PropArgs<T>& a, hipStream_t st)
     {
@@ -100,11 +124,9 @@ struct KS {
     static void step_two(const PropArgs<T>& a, int s, hipStream_t st)
     {
         if constexpr (two_ok) {
-            hipLaunchKernelGGL((k_column2<T, N0, N1, N2, TC>), dim3(ntile, 2 * a.nb), dim3(CG::NT), lds2_col, st, a,
-                               a.nb, a.nb);
+            column_two(a, 2 * a.nb, a.nb, st);
             hipLaunchKernelGGL((k_plane_mid2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, s);
-            hipLaunchKernelGGL((k_column2<T, N0, N1, N2, TC>), dim3(ntile, a.nb), dim3(CG::NT), lds2_col, st, a,
-                               a.nb, 0);
+            column_two(a, a.nb, 0, st);
             hipLaunchKernelGGL((k_plane_end2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, s,
                                (int)(s == a.N - 1 ? PL_LAST : PL_STEP));
         }
@@ -119,7 +141,7 @@ struct KS {
     {
         KernelSet<T> k{N0,      N1,      N2,      &setup_legacy, &begin_legacy,  &step_legacy,
                        nullptr, nullptr, nullptr, "stockham",    &resume_legacy, nullptr,
-                       lds_plane > 80 * 1024};
+                       lds_plane > 80 * 1024, TC, alt_ok ? TC_ALT : 0};
         if (two_ok) {
             k.setup2 = &setup_two;
             k.begin2 = &begin_two;
@@ -369,6 +391,12 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
         step_ = use_two ? ks_->step2 : ks_->step;
         resume_ = use_two ? ks_->resume2 : ks_->resume;
         path_ = use_two ? "two-stage" : "stockham";
+        // SCFT_COLTILE=<width>: the two-stage column pass at the other tile width
+        const char* ct = getenv("SCFT_COLTILE");
+        if (use_two && ct && ks_->tc_alt && atoi(ct) == ks_->tc_alt) {
+            coltile_ = ks_->tc_alt;
+            path_ = ks_->tc_alt == 32 ? "two-stage-c32" : "two-stage-c64";
+        }
         e = setup_();
     } else if (pencil_supported(m[0], m[1], m[2])) {
         begin_ = &pencil_begin<T>;
@@ -420,12 +448,17 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
     if (ensure_streams(nstreams_) != hipSuccess) return "stream / event creation failed";
     // kernel families this grid can run, for the autotuner
     fams_.clear();
-    if (kfound && kfound->step2)
-        fams_.push_back({"two-stage", kfound->setup2, kfound->begin2, kfound->step2, kfound->resume2});
+    if (kfound && kfound->step2) {
+        fams_.push_back({"two-stage", kfound->setup2, kfound->begin2, kfound->step2, kfound->resume2, 0});
+        // the same kernels with the other column tile width
+        if (kfound->tc_alt)
+            fams_.push_back({kfound->tc_alt == 32 ? "two-stage-c32" : "two-stage-c64", kfound->setup2, kfound->begin2,
+                             kfound->step2, kfound->resume2, kfound->tc_alt});
+    }
     if (kfound && kfound->step != kfound->step2)
-        fams_.push_back({"stockham", kfound->setup, kfound->begin, kfound->step, kfound->resume});
+        fams_.push_back({"stockham", kfound->setup, kfound->begin, kfound->step, kfound->resume, 0});
     if (pencil_supported(m[0], m[1], m[2]))
-        fams_.push_back({"pencil", &pencil_setup<T>, &pencil_begin<T>, &pencil_step<T>, &pencil_resume<T>});
+        fams_.push_back({"pencil", &pencil_setup<T>, &pencil_begin<T>, &pencil_step<T>, &pencil_resume<T>, 0});
     return "";
 }
 
@@ -455,6 +488,7 @@ hipError_t PropEngine<T>::tune(const PropArgs<T>& a, int nb, const ChainConst& c
                 begin_ = fams_[f].begin;
                 step_ = fams_[f].step;
                 resume_ = fams_[f].resume;
+                coltile_ = fams_[f].coltile;
                 nstreams_ = ns;
                 use_graph_ = gph == 1;
                 for (auto& g : graphs_) (void)hipGraphExecDestroy(g.exec);
@@ -462,7 +496,9 @@ hipError_t PropEngine<T>::tune(const PropArgs<T>& a, int nb, const ChainConst& c
                 float t = -1;
                 for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
                     if ((e = hipEventRecord(e0, st)) != hipSuccess) break;
-                    if ((e = contour(a, nb, cc, st)) != hipSuccess) break;
+                    PropArgs<T> at = a;
+                    at.coltile = coltile_;
+                    if ((e = contour(at, nb, cc, st)) != hipSuccess) break;
                     if ((e = hipEventRecord(e1, st)) != hipSuccess) break;
                     if ((e = hipEventSynchronize(e1)) != hipSuccess) break;
                     float ms = 0;
@@ -487,6 +523,7 @@ hipError_t PropEngine<T>::tune(const PropArgs<T>& a, int nb, const ChainConst& c
     begin_ = fams_[best.fam].begin;
     step_ = fams_[best.fam].step;
     resume_ = fams_[best.fam].resume;
+    coltile_ = fams_[best.fam].coltile;
     path_ = fams_[best.fam].name;
     nstreams_ = best.streams;
     use_graph_ = best.graph;
@@ -534,7 +571,7 @@ hipError_t PropEngine<T>::run(const double* dW, long long wstride, double* dphi,
                               const int* d_slot2rep, int nb, const ChainConst& cc, double* d_phis, hipStream_t st)
 {
     if (nb <= 0) return hipSuccess;
-    const PropArgs<T> a = args(nb, cc);
+    PropArgs<T> a = args(nb, cc);
 
     const dim3 rg(nblk_, nb);
     hipLaunchKernelGGL(k_wsum, rg, dim3(RED_NT), 0, st, dW, wstride, d_slot2rep, M_, part_);
@@ -545,6 +582,7 @@ hipError_t PropEngine<T>::run(const double* dW, long long wstride, double* dphi,
         tune_pending_ = false;
         hipError_t et = tune(a, nb, cc, st);
         if (et != hipSuccess) return et;
+        a = args(nb, cc);  // the chosen configuration
     }
     if (timing_) (void)hipEventRecord(ev_[0], st);
     hipError_t e = contour(a, nb, cc, st);
@@ -619,6 +657,7 @@ PropArgs<T> PropEngine<T>::args(int nb, const ChainConst& cc) const
     a.Nh = cc.Nh;
     a.nb = nb;
     a.pencil_ct = pencil_ct_ ? 1 : 0;
+    a.coltile = coltile_;
     a.Ch = Ch_;
     a.Cf = Cf_;
     a.e4A = e4A_;

@@ -26,6 +26,8 @@ struct KernelSet {
     void (*resume2)(const PropArgs<T>&, int c, hipStream_t);
     // the LDS Stockham plane needs more than half a CU's LDS (one workgroup per CU)
     bool big_plane;
+    // two-stage column tile: the default width and the alternative (0: none)
+    int tc, tc_alt;
 };
 
 // kernel set for an (m0, m1, m2) grid, or nullptr when no instantiation exists
@@ -93,6 +95,7 @@ private:
         void (*begin)(const PropArgs<T>&, hipStream_t);
         void (*step)(const PropArgs<T>&, int s, hipStream_t);
         void (*resume)(const PropArgs<T>&, int c, hipStream_t);
+        int coltile;  // two-stage column tile width (0: the grid's default)
     };
     std::vector<Family> fams_;
     bool tune_pending_ = false, tuned_ = false;
@@ -125,6 +128,7 @@ private:
     void (*step_)(const PropArgs<T>&, int s, hipStream_t) = nullptr;
     void (*resume_)(const PropArgs<T>&, int c, hipStream_t) = nullptr;
     int ck_ = 0;      // checkpoint interval (0: every contour index kept)
+    int coltile_ = 0;  // two-stage column tile width (0: the grid's default)
     int qpitch_ = 0;  // q-store entries per slot
     const char* path_ = "";
     cpx<T>* Ch_ = nullptr;

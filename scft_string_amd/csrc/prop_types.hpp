@@ -13,6 +13,7 @@ struct PropArgs {
     int N, n1, n2, Nh;     // contour steps, block split, homopolymer length
     int nb;                // active batch slots
     int pencil_ct;         // line kernels: use the compile-time-length ones where they exist
+    int coltile;           // two-stage column pass: tile width (0: the grid's default)
     cpx<T>* Ch;            // [slot][M] half-step sandwich work field
     cpx<T>* Cf;            // [slot][M] full-step sandwich work field
     const T* e4A;          // [slot][M] quarter-step factor of species A (Boltz<T> form)

@@ -221,3 +221,19 @@ def test_two_stage_and_stockham_paths_agree(prec, monkeypatch):
     tol = 1e-12 if prec == "fp64" else 1e-5
     assert _rel(out["0"][0], out["1"][0]) <= tol
     assert abs(out["0"][1][0] - out["1"][1][0]) <= tol * max(1.0, abs(out["1"][1][0]))
+
+
+@pytest.mark.parametrize("m,prec,width", [((64, 64, 64), "fp32", 32), ((48, 48, 48), "fp64", 64),
+                                          ((32, 32, 32), "fp64", 32)])
+def test_column_tile_widths_agree(m, prec, width, monkeypatch):
+    """The two-stage column pass at either tile width (the autotuner may pick either)
+    transforms every column with the same arithmetic: props are bit-identical."""
+    D = (5.0, 5.0, 5.0)
+    W = np.stack([_fields(m, D, seed=30 + i) for i in range(2)])
+    out = {}
+    for ct in ("0", str(width)):
+        monkeypatch.setenv("SCFT_COLTILE", ct)
+        with Context(m, D, 24, 0.45, 0.1, 1, 0.9, chi=15.0, nrep=2, precision=prec) as ctx:
+            assert ctx.fft_path == ("two-stage" if ct == "0" else f"two-stage-c{width}")
+            out[ct] = ctx.props_batch(W)
+    assert np.array_equal(out["0"][0], out[str(width)][0]) and np.array_equal(out["0"][1], out[str(width)][1])
