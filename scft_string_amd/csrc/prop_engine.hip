@@ -261,7 +261,7 @@ static void pencil_px(const PropArgs<T>& a, int nh, int nfields, const PencilTil
 template <typename T>
 static void pencil_begin(const PropArgs<T>& a, hipStream_t st)
 {
-    const PencilTiles p = pencil_tiles(a.m0, a.m1, a.m2, a.pencil_ct);
+    const PencilTiles p = pencil_tiles<T>(a.m0, a.m1, a.m2, a.pencil_ct);
     const size_t lds = pencil_lds_bytes<T>(a.m0, a.m1, a.m2, a.pencil_ct);
     pencil_pz<T>(a, 0, PZ_BEGIN, p, lds, st);
     pencil_py<T, false>(a, a.nb, 2 * a.nb, p, lds, st);
@@ -270,7 +270,7 @@ static void pencil_begin(const PropArgs<T>& a, hipStream_t st)
 template <typename T>
 static void pencil_resume(const PropArgs<T>& a, int c, hipStream_t st)
 {
-    const PencilTiles p = pencil_tiles(a.m0, a.m1, a.m2, a.pencil_ct);
+    const PencilTiles p = pencil_tiles<T>(a.m0, a.m1, a.m2, a.pencil_ct);
     const size_t lds = pencil_lds_bytes<T>(a.m0, a.m1, a.m2, a.pencil_ct);
     pencil_pz<T>(a, c, PZ_RESUME, p, lds, st);
     pencil_py<T, false>(a, a.nb, 2 * a.nb, p, lds, st);
@@ -279,7 +279,7 @@ static void pencil_resume(const PropArgs<T>& a, int c, hipStream_t st)
 template <typename T>
 static void pencil_step(const PropArgs<T>& a, int s, hipStream_t st)
 {
-    const PencilTiles p = pencil_tiles(a.m0, a.m1, a.m2, a.pencil_ct);
+    const PencilTiles p = pencil_tiles<T>(a.m0, a.m1, a.m2, a.pencil_ct);
     const size_t lds = pencil_lds_bytes<T>(a.m0, a.m1, a.m2, a.pencil_ct);
     const bool last = s == a.N - 1;
     pencil_px<T>(a, a.nb, 2 * a.nb, p, lds, st);
@@ -707,7 +707,7 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
     if ((e = hipMemsetAsync(gpart_, 0, sizeof(double) * 3 * np, st)) != hipSuccess) return e;
 
     const PropArgs<T> a = args(nb, cc);
-    const PencilTiles p = pencil_tiles(m_[0], m_[1], m_[2], pencil_ct_);
+    const PencilTiles p = pencil_tiles<T>(m_[0], m_[1], m_[2], pencil_ct_);
     const size_t lds = pencil_lds_bytes<T>(m_[0], m_[1], m_[2], pencil_ct_);
     const double twopi = 8 * atan(1.0);
     const GradConst gc{cc.ds, {twopi / D_[0], twopi / D_[1], twopi / D_[2]}, {D_[0], D_[1], D_[2]}};

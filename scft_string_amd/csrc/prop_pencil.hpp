@@ -94,23 +94,33 @@ __device__ void pc_fft(cpx<T>* buf, int L, int nl, int P, const cpx<T>* tw, int 
 
 // Axis lengths with compile-time kernels: each line pass depends on its own axis length
 // only, so a few lengths cover the large grids (the reference's 200 x 150 x 150 box, FP64
-// 128^3, ...). Their tiles are fixed: nlz = PC_EMAX / 2 / L z lines, t = PC_EMAX / L z
-// columns, and the transform is the compile-time Stockham of fft_engine.hpp (same radix
-// plan and butterflies as pc_fft). Other lengths take the runtime kernels (L = 0).
+// 128^3, ...). Their tiles are fixed: nlz = E / 2 / L z lines, t = E / L z columns with
+// E = pc_ct_emax<T>(L), and the transform is the compile-time Stockham of fft_engine.hpp
+// (same radix plan and butterflies as pc_fft). Other lengths take the runtime kernels
+// (L = 0).
 #define PC_CT_LENGTHS(X) X(96) X(128) X(150) X(160) X(192) X(200) X(256)
 __host__ __device__ constexpr bool pc_ct_len(int L)
 {
     return L == 96 || L == 128 || L == 150 || L == 160 || L == 192 || L == 200 || L == 256;
 }
-__host__ __device__ constexpr int pc_ct_nlz(int L) { return PC_EMAX / 2 / L > 0 ? PC_EMAX / 2 / L : 1; }
-__host__ __device__ constexpr int pc_ct_t(int L) { return PC_EMAX / L > 0 ? PC_EMAX / L : 1; }
+// Elements per workgroup transform of the compile-time kernels: PC_EMAX, except FP64
+// length 96, which takes 1536. Its tiles are then 16 columns (256 B, two cache lines) and
+// 26 KB of LDS, so five workgroups share a CU where four did (96^3 FP64: 0.520 -> 0.474 ms
+// per contour step). Measured on the other lengths, 1536 cost 4-12% at 128 and 256 (runs
+// of 192 and 96 B) and moved 150 x 200 by +-3%.
+template <typename T>
+__host__ __device__ constexpr int pc_ct_emax(int L) { return sizeof(T) == 8 && L == 96 ? 1536 : PC_EMAX; }
+template <typename T>
+__host__ __device__ constexpr int pc_ct_nlz(int L) { return pc_ct_emax<T>(L) / 2 / L > 0 ? pc_ct_emax<T>(L) / 2 / L : 1; }
+template <typename T>
+__host__ __device__ constexpr int pc_ct_t(int L) { return pc_ct_emax<T>(L) / L > 0 ? pc_ct_emax<T>(L) / L : 1; }
 
 // z-pass transform of nfl fields x nlz lines (runtime, or compile-time for L > 0)
 template <typename T, bool INV, int L>
 __device__ __forceinline__ void pz_fft(cpx<T>* buf, int m2, int nfl, int nlz, int P, const cpx<T>* tw, int tid)
 {
     if constexpr (L > 0) {
-        constexpr int NZ = pc_ct_nlz(L);
+        constexpr int NZ = pc_ct_nlz<T>(L);
         if (nfl == 2) fft_lines<T, L, INV, 2 * NZ, 1, pc_pitch(2 * NZ), PC_NT>(buf, tw, tid);
         else fft_lines<T, L, INV, NZ, 1, pc_pitch(NZ), PC_NT>(buf, tw, tid);
     } else {
@@ -122,7 +132,7 @@ __device__ __forceinline__ void pz_fft(cpx<T>* buf, int m2, int nfl, int nlz, in
 template <typename T, bool INV, int L>
 __device__ __forceinline__ void pl_fft(cpx<T>* buf, int len, int nl, int P, const cpx<T>* tw, int tid)
 {
-    if constexpr (L > 0) fft_lines<T, L, INV, pc_ct_t(L), 1, pc_pitch(pc_ct_t(L)), PC_NT>(buf, tw, tid);
+    if constexpr (L > 0) fft_lines<T, L, INV, pc_ct_t<T>(L), 1, pc_pitch(pc_ct_t<T>(L)), PC_NT>(buf, tw, tid);
     else pc_fft<T, INV>(buf, len, nl, P, tw, tid);
 }
 
@@ -160,7 +170,7 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
 {
     extern __shared__ unsigned char smem_raw[];
     const int m2 = L > 0 ? L : a.m2;
-    const int nlz = L > 0 ? pc_ct_nlz(L) : nlz_arg;
+    const int nlz = L > 0 ? pc_ct_nlz<T>(L) : nlz_arg;
     const int nfl = mode == PZ_MID ? 1 : 2;
     const bool fresh = mode == PZ_BEGIN || mode == PZ_RESUME;  // q not computed from Ch, Cf
     const int nlines = nfl * nlz, P = pc_pitch(nlines);
@@ -287,7 +297,7 @@ __global__ void __launch_bounds__(PC_NT) k_py(PropArgs<T> a, int nh, int tz_arg)
 {
     extern __shared__ unsigned char smem_raw[];
     const int m1 = L > 0 ? L : a.m1, m2 = a.m2;
-    const int tz = L > 0 ? pc_ct_t(L) : tz_arg, P = pc_pitch(tz);
+    const int tz = L > 0 ? pc_ct_t<T>(L) : tz_arg, P = pc_pitch(tz);
     cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
     cpx<T>* tw = buf + (size_t)m1 * P;
     const int tid = threadIdx.x;
@@ -329,7 +339,7 @@ __global__ void __launch_bounds__(PC_NT) k_px(PropArgs<T> a, int nh, int tx_arg)
 {
     extern __shared__ unsigned char smem_raw[];
     const int m0 = L > 0 ? L : a.m0, m2 = a.m2;
-    const int tx = L > 0 ? pc_ct_t(L) : tx_arg, P = pc_pitch(tx);
+    const int tx = L > 0 ? pc_ct_t<T>(L) : tx_arg, P = pc_pitch(tx);
     const size_t sx = (size_t)a.m1 * m2;
     cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
     cpx<T>* tw = buf + (size_t)m0 * P;
@@ -384,6 +394,7 @@ struct PencilTiles {
 
 // Tiling for an m0 x m1 x m2 grid; every transform stays within PC_EMAX elements. With ct,
 // axes of a compile-time length take that kernel's fixed tile.
+template <typename T>
 inline PencilTiles pencil_tiles(int m0, int m1, int m2, bool ct)
 {
     auto tile = [](int len, int cap) {
@@ -394,10 +405,10 @@ inline PencilTiles pencil_tiles(int m0, int m1, int m2, bool ct)
         return (cap + n - 1) / n;
     };
     PencilTiles p;
-    p.nlz = PC_EMAX / 2 / m2;
+    p.nlz = (ct && pc_ct_len(m2)) ? pc_ct_nlz<T>(m2) : PC_EMAX / 2 / m2;
     if (p.nlz < 1) p.nlz = 1;
-    p.ty = (ct && pc_ct_len(m1)) ? pc_ct_t(m1) : tile(m1, m2);
-    p.tx = (ct && pc_ct_len(m0)) ? pc_ct_t(m0) : tile(m0, m2);
+    p.ty = (ct && pc_ct_len(m1)) ? pc_ct_t<T>(m1) : tile(m1, m2);
+    p.tx = (ct && pc_ct_len(m0)) ? pc_ct_t<T>(m0) : tile(m0, m2);
     return p;
 }
 
@@ -423,7 +434,7 @@ constexpr size_t pencil_lds_max()
 template <typename T>
 inline size_t pencil_lds_bytes(int m0, int m1, int m2, bool ct)
 {
-    const PencilTiles p = pencil_tiles(m0, m1, m2, ct);
+    const PencilTiles p = pencil_tiles<T>(m0, m1, m2, ct);
     const size_t z = (size_t)m2 * pc_pitch(2 * p.nlz) + m2;
     const size_t y = (size_t)m1 * pc_pitch(p.ty) + m1;
     const size_t x = (size_t)m0 * pc_pitch(p.tx) + m0;
