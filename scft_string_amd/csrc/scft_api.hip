@@ -125,6 +125,19 @@ scft_status finalize_to_host(scft_ctx* c, int nslots, int ncomp, std::vector<dou
     return SCFT_OK;
 }
 
+// solve_field kills the run when its error norm is NaN (assert(err==err), scft.cu:770);
+// here the iteration returns SCFT_ERR_NAN before the fields are mixed.
+scft_status check_errs_finite(scft_ctx* c, const double* errs, int R)
+{
+    for (int ii = 0; ii < R; ++ii)
+        if (!(errs[ii] == errs[ii])) {
+            char b[128];
+            snprintf(b, sizeof b, "solve_field: error norm is NaN for replica %d", ii);
+            return fail(c, SCFT_ERR_NAN, b);
+        }
+    return SCFT_OK;
+}
+
 std::vector<int> fixed_doiis(const scft_ctx* c)
 {
     std::vector<int> d(c->R, 1);  // single process owns every replica (scft.cu:952-975)
@@ -234,6 +247,7 @@ scft_status anderson_iterate(scft_ctx* c, double* alf, scft_iter_report* rep)
             errs[active[j]] = std::sqrt(devs[2 * j] / (double)M2);
             steps[active[j]] = 1;
         }
+        SCFTCHK(check_errs_finite(c, errs.data(), R));
     }
     for (const int ii : active) {
         double* W = c->dW + (size_t)ii * M2;
@@ -623,6 +637,7 @@ scft_status scft_string_iterate(scft_ctx* c, int k, double errTol, double* alf, 
         sdevm[active[j]] = devs[2 * j + 1];
     }
     for (int ii = 0; ii < R; ++ii) errs[ii] = std::pow(sdev[ii] / (2 * M), 0.5);
+    SCFTCHK(check_errs_finite(c, errs.data(), R));
 
     std::vector<int> proj(R, 0);  // interior replicas whose DEV- is projected (scft.cu:670-678)
     for (int ii = 0; ii < R; ++ii) c->hsmall[3 * R + ii] = 0.0;  // cproj
