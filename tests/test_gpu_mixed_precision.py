@@ -9,12 +9,19 @@ phidb (density.h:405-422), so F drifts with the uniform part of W+, which canoni
 relaxation never removes. In the grand-canonical ensemble the reference's F uses
 Q2 + phidb*QC for lnQ (density.h:398), which is stationary only up to that same
 discretisation, so its bar is 1e-5."""
+import sys
+
 import numpy as np
 import pytest
 
 from scft_string_amd import Context
 
 pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
 
 
 def _string(m, D, R, seed=3):
@@ -65,3 +72,33 @@ def test_free_energy_fp32_context_matches_fp64(ens, f, bar):
           f"all-FP32 FreeE |dF| max {d1.max():.2e}", flush=True)
     assert d.max() <= bar
     assert between.max() <= bar
+
+
+def test_full_size_string_free_energy_fp32_within_1e6():
+    """The benchmark string (16 replicas, 64^3, N = 128, chiN = 18, f = 0.45, phidb = 0.9,
+    canonical, fix = (1, 1), lamellar -> cylinder ends), 3 iterations with the FP32
+    propagator; then FreeE on those identical fields in the FP32 context (W+ relaxed with FP32
+    props, F from an FP64 props: fe_fp64) and in an FP64 context. |dF| <= 1e-6 per replica and
+    between replicas. The all-FP32 FreeE (fe_fp64 = 0) is reported, not bounded."""
+    from bench import CFG, initial_fields
+
+    c = CFG
+    R = c["nrep"]
+    with Context(c["m"], c["D"], c["N"], c["f"], c["alpha"], c["ens"], c["phidb"], c["chi"], nrep=R,
+                 precision="fp32", fix=c["fix"]) as ctx:
+        ctx.set_fields(initial_fields(c["m"], c["D"], R))
+        for k in range(3):
+            ctx.string_iterate(k)
+        W = ctx.get_fields()
+    fe = {}
+    for prec, fe64 in (("fp32", 1), ("fp64", 1), ("fp32", 0)):
+        with Context(c["m"], c["D"], c["N"], c["f"], c["alpha"], c["ens"], c["phidb"], c["chi"], nrep=R,
+                     precision=prec, fix=c["fix"], fe_fp64=fe64) as ctx:
+            ctx.set_fields(W)
+            fe[(prec, fe64)] = ctx.free_energy()["FEs"]
+    d = np.abs(fe[("fp32", 1)] - fe[("fp64", 1)])
+    rel = (fe[("fp32", 1)] - fe[("fp32", 1)][0]) - (fe[("fp64", 1)] - fe[("fp64", 1)][0])
+    d32 = np.abs(fe[("fp32", 0)] - fe[("fp64", 1)])
+    _say(f"16 x 64^3 string: |dF| {d.max():.2e}, between replicas {np.abs(rel).max():.2e}; "
+         f"all-FP32 FreeE |dF| {d32.max():.2e}")
+    assert d.max() <= 1e-6 and np.abs(rel).max() <= 1e-6
