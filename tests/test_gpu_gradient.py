@@ -192,3 +192,44 @@ def test_autograd_backward_of_free_energy():
     assert _rel(dD1[0], dFdD[0]) <= 1e-10 and np.all(dD1[1] == 0) and dchi1[1] == 0
     # sum_r dF_r/dD_x: both replicas carry the oracle's stress -d lnQ/dD_x
     assert _rel(Dt.grad.numpy()[0], -(wts[0] + wts[1]) * dl[0]) <= 1e-10
+
+
+@pytest.mark.parametrize("D0", [5.6, 6.0, 6.4])
+def test_gradient_at_64cube_N100(D0):
+    """The adjoint workload's own size: 64^3, N = 100, chiN = 20, f = 0.5, a lamella-to-cylinder
+    field, D swept over 5.6-6.4. FP64 d lnQ/dD against the oracle (1e-10) and against central
+    differences with h = 1e-4 (1e-6: the difference quotient's own error is O(h^2));
+    checkpoints every 10 contour steps against the full store (1e-12); FP32 against FP64
+    (1e-4)."""
+    m, N, f, alpha, ens, phidb = (64, 64, 64), 100, 0.5, 0.1, 1, 0.8
+    D = (D0, D0, D0)
+    M = int(np.prod(m))
+    X, Y = np.meshgrid(2 * np.pi * np.arange(m[0]) / m[0], 2 * np.pi * np.arange(m[1]) / m[1], indexing="ij")
+    lam = np.cos(X)
+    cyl = (np.cos(X) + np.cos(X / 2 + np.sqrt(3) * Y / 2) + np.cos(X / 2 - np.sqrt(3) * Y / 2)) / 3
+    W = np.zeros((1, 2, M))
+    W[0, 0] = np.repeat((0.6 * lam + 0.4 * cyl)[:, :, None] * 1.5, m[2], axis=2).reshape(-1)
+    res = {}
+    for prec, ck in (("fp64", 0), ("fp64", 10), ("fp32", 0)):
+        with Context(m, D, N, f, alpha, ens, phidb, chi=20.0, precision=prec, dostring=0, ckpt=ck) as ctx:
+            lq, dq = ctx.dlnq_dbox(W)
+            res[(prec, ck)] = (lq[0], dq[0])
+            if prec == "fp64" and ck == 0:
+                h, fd = 1e-4, np.zeros(3)
+                for i in range(3):
+                    lp = []
+                    for s in (1, -1):
+                        Dh = list(D)
+                        Dh[i] += s * h
+                        ctx.set_box(Dh)
+                        lp.append(ctx.props_batch(W)[1][0])
+                    fd[i] = (lp[0] - lp[1]) / (2 * h)
+                ctx.set_box(D)
+    orc.set_threads(16)
+    _, dl_o = orc.Grid(m, D).dlnq_dbox(W[0], N, alpha, phidb, f, ens)
+    g64, g64c, g32 = res[("fp64", 0)][1], res[("fp64", 10)][1], res[("fp32", 0)][1]
+    _say(f"D={D0}: gpu {g64}, oracle {dl_o}, fd {fd}; ckpt10 rel {_rel(g64c, g64):.1e}, fp32 rel {_rel(g32, g64):.1e}")
+    assert _rel(g64, dl_o) <= 1e-10
+    assert _rel(g64, fd) <= 1e-6
+    assert _rel(g64c, g64) <= 1e-12
+    assert _rel(g32, g64) <= 1e-4
