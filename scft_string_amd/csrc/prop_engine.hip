@@ -5,6 +5,7 @@
 #include "prop_pencil.hpp"
 #include "grad_kernels.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -707,13 +708,18 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
     if ((e = hipMemsetAsync(gpart_, 0, sizeof(double) * 3 * np, st)) != hipSuccess) return e;
 
     const PropArgs<T> a = args(nb, cc);
+    // pencil_py runs on the props tiling (compiled lengths where they exist); k_gz and k_gx
+    // are runtime-length kernels sized for PC_EMAX elements, so they take the runtime tiling.
+    // Each pass covers the whole field, so the two tilings mix freely.
     const PencilTiles p = pencil_tiles<T>(m_[0], m_[1], m_[2], pencil_ct_);
-    const size_t lds = pencil_lds_bytes<T>(m_[0], m_[1], m_[2], pencil_ct_);
+    const PencilTiles pg = pencil_tiles<T>(m_[0], m_[1], m_[2], false);
+    const size_t lds = std::max(pencil_lds_bytes<T>(m_[0], m_[1], m_[2], pencil_ct_),
+                                pencil_lds_bytes<T>(m_[0], m_[1], m_[2], false));
     const double twopi = 8 * atan(1.0);
     const GradConst gc{cc.ds, {twopi / D_[0], twopi / D_[1], twopi / D_[2]}, {D_[0], D_[1], D_[2]}};
     const long long lines = (long long)m_[0] * m_[1];
-    const dim3 gzg((unsigned)((lines + p.nlz - 1) / p.nlz), nb);
-    const int ntx = (m_[2] + p.tx - 1) / p.tx;
+    const dim3 gzg((unsigned)((lines + pg.nlz - 1) / pg.nlz), nb);
+    const int ntx = (m_[2] + pg.tx - 1) / pg.tx;
     // The term of contour step s of a chain, b and a read from q-store entries ib, ia:
     //   chain 0: b = q1[s], a = q2[N-1-s], species of chain 1 on step s (density.h:136-146)
     //   chain 1: b = q2[t], a = q2[Nh-1-t], species of chain 2 on step t
@@ -722,14 +728,14 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
         double* pc = gpart_ + (size_t)chain * np;
         const int pbk = chain == 1 ? 1 : 0, pak = chain == 2 ? 0 : 1;
         const int spec = chain == 1 ? (s < cc.n2 ? 1 : 0) : (s < cc.n1 ? 0 : 1);
-        hipLaunchKernelGGL((k_gz<T>), gzg, dim3(PC_NT), lds, st, a, (int)GZ_INIT, ib, pbk, ia, pak, spec, p.nlz);
+        hipLaunchKernelGGL((k_gz<T>), gzg, dim3(PC_NT), lds, st, a, (int)GZ_INIT, ib, pbk, ia, pak, spec, pg.nlz);
         pencil_py<T, false>(a, nb, 2 * nb, p, lds, st);
-        hipLaunchKernelGGL((k_gx<T>), dim3(m_[1] * ntx, 2 * nb), dim3(PC_NT), lds, st, a, (int)GX_SPLIT, nb, p.tx,
+        hipLaunchKernelGGL((k_gx<T>), dim3(m_[1] * ntx, 2 * nb), dim3(PC_NT), lds, st, a, (int)GX_SPLIT, nb, pg.tx,
                            K0_, Kf_);
         pencil_py<T, true>(a, nb, nb, p, lds, st);
-        hipLaunchKernelGGL((k_gz<T>), gzg, dim3(PC_NT), lds, st, a, (int)GZ_MID, 0, 0, 0, 0, spec, p.nlz);
+        hipLaunchKernelGGL((k_gz<T>), gzg, dim3(PC_NT), lds, st, a, (int)GZ_MID, 0, 0, 0, 0, spec, pg.nlz);
         pencil_py<T, false>(a, nb, nb, p, lds, st);
-        hipLaunchKernelGGL((k_gx<T>), dim3(m_[1] * ntx, nb), dim3(PC_NT), lds, st, a, (int)GX_FWD, nb, p.tx, K1_,
+        hipLaunchKernelGGL((k_gx<T>), dim3(m_[1] * ntx, nb), dim3(PC_NT), lds, st, a, (int)GX_FWD, nb, pg.tx, K1_,
                            (cpx<T>*)nullptr);
         hipLaunchKernelGGL((k_gacc<T>), dim3(nblk_, nb), dim3(RED_NT), 0, st, K0_, K1_, Kf_, M_, m_[0], m_[1],
                            m_[2], gc, pc);

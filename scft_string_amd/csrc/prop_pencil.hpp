@@ -106,10 +106,15 @@ __host__ __device__ constexpr bool pc_ct_len(int L)
 // Elements per workgroup transform of the compile-time kernels: PC_EMAX, except FP64
 // length 96, which takes 1536. Its tiles are then 16 columns (256 B, two cache lines) and
 // 26 KB of LDS, so five workgroups share a CU where four did (96^3 FP64: 0.520 -> 0.474 ms
-// per contour step). Measured on the other lengths, 1536 cost 4-12% at 128 and 256 (runs
-// of 192 and 96 B) and moved 150 x 200 by +-3%.
+// per contour step). Measured on MI355X: 1536 on the other FP64 lengths cost 4-12% at 128
+// and 256 (runs of 192 and 96 B) and moved 150 x 200 by +-3%; 4096 for FP32 (twice the
+// columns and lines per tile) doubled the registers, cut occupancy from 7 to 3 waves per
+// SIMD and slowed the input.dat box 25%.
 template <typename T>
-__host__ __device__ constexpr int pc_ct_emax(int L) { return sizeof(T) == 8 && L == 96 ? 1536 : PC_EMAX; }
+__host__ __device__ constexpr int pc_ct_emax(int L)
+{
+    return sizeof(T) == 8 && L == 96 ? 1536 : PC_EMAX;
+}
 template <typename T>
 __host__ __device__ constexpr int pc_ct_nlz(int L) { return pc_ct_emax<T>(L) / 2 / L > 0 ? pc_ct_emax<T>(L) / 2 / L : 1; }
 template <typename T>
@@ -155,6 +160,12 @@ __device__ __forceinline__ int pc_xcd_block()
 // loops, so all of a thread's loads are in flight together before the first LDS write.
 constexpr int PZ_EPT = PC_EMAX / 2 / PC_NT;  // per field in k_pz (nlz m2 <= PC_EMAX / 2)
 constexpr int PC_EPT = PC_EMAX / PC_NT;      // in k_py, k_px (lines x length <= PC_EMAX)
+// the same for a kernel of compile-time length L (L = 0: the runtime plan, PC_EMAX)
+template <typename T, int L>
+__host__ __device__ constexpr int pc_ept()
+{
+    return (L > 0 ? pc_ct_emax<T>(L) : PC_EMAX) / PC_NT;
+}
 
 // z-line pass. grid (ceil(m0 m1 / nlz), nb slots), block PC_NT.
 //   PZ_MID:   Ch <- Fz(E1/2 Fz^-1(Ch))                         (density.h:156-173)
@@ -168,6 +179,7 @@ constexpr int PC_EPT = PC_EMAX / PC_NT;      // in k_py, k_px (lines x length <=
 template <typename T, int L = 0>
 __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, int nlz_arg)
 {
+    constexpr int ZE = pc_ept<T, L>() / 2;  // elements per thread and field
     extern __shared__ unsigned char smem_raw[];
     const int m2 = L > 0 ? L : a.m2;
     const int nlz = L > 0 ? pc_ct_nlz<T>(L) : nlz_arg;
@@ -183,10 +195,10 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
     const size_t fo = (size_t)slot * (size_t)a.M + (size_t)l0 * m2;
     const int ne = nl * m2, nt = nlz * m2;  // valid elements, tile elements
 
-    T va[PZ_EPT], vb[PZ_EPT];
-    cpx<T> c0[PZ_EPT], c1[PZ_EPT];
+    T va[ZE], vb[ZE];
+    cpx<T> c0[ZE], c1[ZE];
 #pragma unroll
-    for (int i = 0; i < PZ_EPT; ++i) {
+    for (int i = 0; i < ZE; ++i) {
         const int e = tid + i * PC_NT;
         const bool ok = e < ne;
         va[i] = ok ? a.e4A[fo + e] : T(0);
@@ -198,7 +210,7 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
 
     if (!fresh) {
 #pragma unroll
-        for (int i = 0; i < PZ_EPT; ++i) {
+        for (int i = 0; i < ZE; ++i) {
             const int e = tid + i * PC_NT;
             if (e < nt) {
                 const int l = e / m2, z = e - l * m2;
@@ -213,7 +225,7 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
     if (mode == PZ_MID) {
         const bool qa = s < a.n1, pb = s < a.n2;  // species of this step (density.h:136-146)
 #pragma unroll
-        for (int i = 0; i < PZ_EPT; ++i) {
+        for (int i = 0; i < ZE; ++i) {
             const int e = tid + i * PC_NT;
             if (e < ne) {
                 const int l = e / m2, z = e - l * m2;
@@ -225,7 +237,7 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
         __syncthreads();
         pz_fft<T, false, L>(buf, m2, nfl, nlz, P, tw, tid);
 #pragma unroll
-        for (int i = 0; i < PZ_EPT; ++i) {
+        for (int i = 0; i < ZE; ++i) {
             const int e = tid + i * PC_NT;
             if (e < ne) {
                 const int l = e / m2, z = e - l * m2;
@@ -242,7 +254,7 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
     const bool qa = s < a.n1, pb = s < a.n2;        // this step's species
     const bool na = sout < a.n1, nbb = sout < a.n2;  // next step's species
 #pragma unroll
-    for (int i = 0; i < PZ_EPT; ++i) {
+    for (int i = 0; i < ZE; ++i) {
         const int e = tid + i * PC_NT;
         if (e < nt) {
             const int l = e / m2, z = e - l * m2;
@@ -273,7 +285,7 @@ __global__ void __launch_bounds__(PC_NT) k_pz(PropArgs<T> a, int s, int mode, in
     __syncthreads();
     pz_fft<T, false, L>(buf, m2, nfl, nlz, P, tw, tid);
 #pragma unroll
-    for (int i = 0; i < PZ_EPT; ++i) {
+    for (int i = 0; i < ZE; ++i) {
         const int e = tid + i * PC_NT;
         if (e < ne) {
             const int l = e / m2, z = e - l * m2;
@@ -295,6 +307,7 @@ __device__ __forceinline__ cpx<T>* pc_field(const PropArgs<T>& a, int f, int nh)
 template <typename T, bool INV, int L = 0>
 __global__ void __launch_bounds__(PC_NT) k_py(PropArgs<T> a, int nh, int tz_arg)
 {
+    constexpr int CE = pc_ept<T, L>();  // elements per thread
     extern __shared__ unsigned char smem_raw[];
     const int m1 = L > 0 ? L : a.m1, m2 = a.m2;
     const int tz = L > 0 ? pc_ct_t<T>(L) : tz_arg, P = pc_pitch(tz);
@@ -307,16 +320,16 @@ __global__ void __launch_bounds__(PC_NT) k_py(PropArgs<T> a, int nh, int tz_arg)
     const int nz = m2 - z0 < tz ? m2 - z0 : tz;
     cpx<T>* g = pc_field(a, blockIdx.y, nh) + (size_t)x * m1 * m2 + z0;
     const int ne = m1 * tz;
-    cpx<T> r[PC_EPT];
+    cpx<T> r[CE];
 #pragma unroll
-    for (int i = 0; i < PC_EPT; ++i) {
+    for (int i = 0; i < CE; ++i) {
         const int e = tid + i * PC_NT;
         const int y = e / tz, zz = e - y * tz;
         r[i] = (e < ne && zz < nz) ? g[(size_t)y * m2 + zz] : cpx<T>{T(0), T(0)};
     }
     for (int i = tid; i < m1; i += PC_NT) tw[i] = a.tw1[i];
 #pragma unroll
-    for (int i = 0; i < PC_EPT; ++i) {
+    for (int i = 0; i < CE; ++i) {
         const int e = tid + i * PC_NT;
         const int y = e / tz, zz = e - y * tz;
         if (e < ne) buf[y * P + zz] = r[i];
@@ -324,7 +337,7 @@ __global__ void __launch_bounds__(PC_NT) k_py(PropArgs<T> a, int nh, int tz_arg)
     __syncthreads();
     pl_fft<T, INV, L>(buf, m1, tz, P, tw, tid);
 #pragma unroll
-    for (int i = 0; i < PC_EPT; ++i) {
+    for (int i = 0; i < CE; ++i) {
         const int e = tid + i * PC_NT;
         const int y = e / tz, zz = e - y * tz;
         if (e < ne && zz < nz) g[(size_t)y * m2 + zz] = buf[y * P + zz];
@@ -337,6 +350,7 @@ __global__ void __launch_bounds__(PC_NT) k_py(PropArgs<T> a, int nh, int tz_arg)
 template <typename T, int L = 0>
 __global__ void __launch_bounds__(PC_NT) k_px(PropArgs<T> a, int nh, int tx_arg)
 {
+    constexpr int CE = pc_ept<T, L>();  // elements per thread
     extern __shared__ unsigned char smem_raw[];
     const int m0 = L > 0 ? L : a.m0, m2 = a.m2;
     const int tx = L > 0 ? pc_ct_t<T>(L) : tx_arg, P = pc_pitch(tx);
@@ -351,16 +365,16 @@ __global__ void __launch_bounds__(PC_NT) k_px(PropArgs<T> a, int nh, int tx_arg)
     const bool full = (int)blockIdx.y >= nh;
     cpx<T>* g = pc_field(a, blockIdx.y, nh) + (size_t)y * m2 + z0;
     const int ne = m0 * tx;
-    cpx<T> r[PC_EPT];
+    cpx<T> r[CE];
 #pragma unroll
-    for (int i = 0; i < PC_EPT; ++i) {
+    for (int i = 0; i < CE; ++i) {
         const int e = tid + i * PC_NT;
         const int x = e / tx, zz = e - x * tx;
         r[i] = (e < ne && zz < nz) ? g[x * sx + zz] : cpx<T>{T(0), T(0)};
     }
     for (int i = tid; i < m0; i += PC_NT) tw[i] = a.tw0[i];
 #pragma unroll
-    for (int i = 0; i < PC_EPT; ++i) {
+    for (int i = 0; i < CE; ++i) {
         const int e = tid + i * PC_NT;
         const int x = e / tx, zz = e - x * tx;
         if (e < ne) buf[x * P + zz] = r[i];
@@ -369,7 +383,7 @@ __global__ void __launch_bounds__(PC_NT) k_px(PropArgs<T> a, int nh, int tx_arg)
     pl_fft<T, false, L>(buf, m0, tx, P, tw, tid);
     const T gy = a.gy[y];
 #pragma unroll
-    for (int i = 0; i < PC_EPT; ++i) {
+    for (int i = 0; i < CE; ++i) {
         const int e = tid + i * PC_NT;
         const int x = e / tx, zz = e - x * tx;
         if (e < ne && zz < nz)
@@ -378,7 +392,7 @@ __global__ void __launch_bounds__(PC_NT) k_px(PropArgs<T> a, int nh, int tx_arg)
     __syncthreads();
     pl_fft<T, true, L>(buf, m0, tx, P, tw, tid);
 #pragma unroll
-    for (int i = 0; i < PC_EPT; ++i) {
+    for (int i = 0; i < CE; ++i) {
         const int e = tid + i * PC_NT;
         const int x = e / tx, zz = e - x * tx;
         if (e < ne && zz < nz) g[x * sx + zz] = buf[x * P + zz];
