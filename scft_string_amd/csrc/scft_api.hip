@@ -125,6 +125,19 @@ scft_status finalize_to_host(scft_ctx* c, int nslots, int ncomp, std::vector<dou
     return SCFT_OK;
 }
 
+// HIP's current device belongs to the host thread. A context may be created on one thread
+// and used from others (scft_iterate_concurrent runs each context on its own thread), so
+// every entry point makes the context's device current before it touches the context's
+// memory or streams.
+scft_status bind_device(scft_ctx* c)
+{
+    int d = -1;
+    if (hipGetDevice(&d) == hipSuccess && d == c->p.device) return SCFT_OK;
+    const hipError_t e = hipSetDevice(c->p.device);
+    if (e != hipSuccess) return fail(c, SCFT_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    return SCFT_OK;
+}
+
 // solve_field kills the run when its error norm is NaN (assert(err==err), scft.cu:770);
 // here the iteration returns SCFT_ERR_NAN before the fields are mixed.
 scft_status check_errs_finite(scft_ctx* c, const double* errs, int R)
@@ -462,6 +475,7 @@ scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
 scft_status scft_ctx_destroy(scft_ctx* c)
 {
     if (!c) return SCFT_OK;
+    (void)bind_device(c);
     if (c->st) (void)hipStreamSynchronize(c->st);
     c->pe32.reset();
     c->pe64.reset();
@@ -482,6 +496,7 @@ scft_status scft_ctx_destroy(scft_ctx* c)
 scft_status scft_sync(scft_ctx* c)
 {
     if (!c) return SCFT_ERR_INVALID;
+    SCFTCHK(bind_device(c));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return SCFT_OK;
 }
@@ -489,6 +504,7 @@ scft_status scft_sync(scft_ctx* c)
 scft_status scft_props_batch(scft_ctx* c, const double* W, double* phi, double* lnQ, double* phis, int nrep)
 {
     if (!c || !W || nrep <= 0 || nrep > c->R) return fail(c, SCFT_ERR_INVALID, "scft_props_batch: bad arguments");
+    SCFTCHK(bind_device(c));
     const long long M = c->M;
     HIPCHK(c, hipMemcpyAsync(c->dWs, W, sizeof(double) * 2 * M * nrep, hipMemcpyHostToDevice, c->st));
     std::vector<int> slots(nrep);
@@ -507,6 +523,7 @@ scft_status scft_props_batch_device(scft_ctx* c, const double* dW, double* dphi,
 {
     if (!c || !dW || !dphi || nrep <= 0 || nrep > c->R)
         return fail(c, SCFT_ERR_INVALID, "scft_props_batch_device: bad arguments");
+    SCFTCHK(bind_device(c));
     std::vector<int> slots(nrep);
     for (int i = 0; i < nrep; ++i) slots[i] = i;
     std::vector<double> lq(nrep), ph(5 * (size_t)nrep);
@@ -519,6 +536,7 @@ scft_status scft_props_batch_device(scft_ctx* c, const double* dW, double* dphi,
 scft_status scft_set_fields(scft_ctx* c, const double* W)
 {
     if (!c || !W) return fail(c, SCFT_ERR_INVALID, "scft_set_fields: null");
+    SCFTCHK(bind_device(c));
     HIPCHK(c, hipMemcpyAsync(c->dW, W, sizeof(double) * 2 * c->M * c->R, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     anderson_reset(c);
@@ -528,6 +546,7 @@ scft_status scft_set_fields(scft_ctx* c, const double* W)
 scft_status scft_get_fields(scft_ctx* c, double* W)
 {
     if (!c || !W) return fail(c, SCFT_ERR_INVALID, "scft_get_fields: null");
+    SCFTCHK(bind_device(c));
     HIPCHK(c, hipMemcpyAsync(W, c->dW, sizeof(double) * 2 * c->M * c->R, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return SCFT_OK;
@@ -536,6 +555,7 @@ scft_status scft_get_fields(scft_ctx* c, double* W)
 scft_status scft_get_densities(scft_ctx* c, double* phi)
 {
     if (!c || !phi) return fail(c, SCFT_ERR_INVALID, "scft_get_densities: null");
+    SCFTCHK(bind_device(c));
     HIPCHK(c, hipMemcpyAsync(phi, c->dphi, sizeof(double) * 4 * c->M * c->R, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return SCFT_OK;
@@ -544,6 +564,7 @@ scft_status scft_get_densities(scft_ctx* c, double* phi)
 scft_status scft_get_tangent(scft_ctx* c, double* dWda)
 {
     if (!c || !dWda) return fail(c, SCFT_ERR_INVALID, "scft_get_tangent: null");
+    SCFTCHK(bind_device(c));
     HIPCHK(c, hipMemcpyAsync(dWda, c->ddWda, sizeof(double) * c->M * c->R, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return SCFT_OK;
@@ -553,6 +574,7 @@ scft_status scft_get_tangent(scft_ctx* c, double* dWda)
 scft_status scft_solve_w_plus(scft_ctx* c, int maxIter, double errTol, const int* which, int* k_out)
 {
     if (!c) return SCFT_ERR_INVALID;
+    SCFTCHK(bind_device(c));
     anderson_reset(c);  // W+ changes outside the Anderson iteration
     const int R = c->R;
     const long long M = c->M;
@@ -597,6 +619,7 @@ scft_status scft_solve_w_plus(scft_ctx* c, int maxIter, double errTol, const int
 scft_status scft_string_iterate(scft_ctx* c, int k, double errTol, double* alf, scft_iter_report* rep)
 {
     if (!c) return SCFT_ERR_INVALID;
+    SCFTCHK(bind_device(c));
     if (c->p.anderson > 0) return anderson_iterate(c, alf, rep);
     const int R = c->R;
     const long long M = c->M;
@@ -740,6 +763,7 @@ scft_status scft_free_energy(scft_ctx* c, double* FE, double* FEs, double* FEs0,
                              double* alf)
 {
     if (!c) return SCFT_ERR_INVALID;
+    SCFTCHK(bind_device(c));
     const int R = c->R;
     const long long M = c->M;
     const std::vector<int> doiis = fixed_doiis(c);
@@ -840,6 +864,7 @@ scft_status scft_dlnq_dbox(scft_ctx* c, const double* W, int nrow, double* lnQ, 
 {
     if (!c || !W || !dlnQ_dD || nrow <= 0 || nrow > c->R)
         return fail(c, SCFT_ERR_INVALID, "scft_dlnq_dbox: bad arguments");
+    SCFTCHK(bind_device(c));
     const long long M = c->M;
     HIPCHK(c, hipMemcpyAsync(c->dWs, W, sizeof(double) * 2 * M * nrow, hipMemcpyHostToDevice, c->st));
     std::vector<int> slots(nrow);
@@ -854,6 +879,7 @@ scft_status scft_dlnq_dbox(scft_ctx* c, const double* W, int nrow, double* lnQ, 
 scft_status scft_box_stress(scft_ctx* c, double* dFdD, double* dFdchi)
 {
     if (!c) return SCFT_ERR_INVALID;
+    SCFTCHK(bind_device(c));
     const int R = c->R;
     const long long M = c->M;
     std::vector<int> slots(R);
@@ -895,6 +921,7 @@ scft_status scft_set_box(scft_ctx* c, const double D[3])
 {
     if (!c || !D || !(D[0] > 0 && D[1] > 0 && D[2] > 0))
         return fail(c, SCFT_ERR_INVALID, "scft_set_box: box lengths must be positive");
+    SCFTCHK(bind_device(c));
     HIPCHK(c, hipStreamSynchronize(c->st));
     hipError_t e = hipSuccess;
     if (c->pe32) e = c->pe32->set_box(D);
@@ -977,6 +1004,7 @@ scft_status scft_relax_box(scft_ctx* c, int axes, double stressTol, double errTo
 scft_status scft_debug_qq_invariant(scft_ctx* c, int slot, double* qq)
 {
     if (!c || !qq || slot < 0 || slot >= c->R) return SCFT_ERR_INVALID;
+    SCFTCHK(bind_device(c));
     hipError_t e = c->pe32 ? c->pe32->qq_invariant(slot, qq, c->st) : c->pe64->qq_invariant(slot, qq, c->st);
     if (e != hipSuccess) return fail(c, SCFT_ERR_HIP, hipGetErrorString(e));
     return SCFT_OK;
@@ -985,6 +1013,7 @@ scft_status scft_debug_qq_invariant(scft_ctx* c, int slot, double* qq)
 scft_status scft_debug_time_props(scft_ctx* c, int nrep, int reps, double* ms_per_step)
 {
     if (!c || !ms_per_step || nrep <= 0 || nrep > c->R || reps <= 0) return SCFT_ERR_INVALID;
+    SCFTCHK(bind_device(c));
     std::vector<int> slots(nrep);
     for (int i = 0; i < nrep; ++i) slots[i] = i;
     hipError_t e = c->pe32 ? c->pe32->enable_timing(true) : c->pe64->enable_timing(true);
