@@ -6,7 +6,9 @@ The string run repeats main()'s order (scft.cu:1102-1155):
 - a final FreeE.
 The oracle runs the same sequence from the same win files, and the FEs file (10 decimals,
 scft.cu:843-850) must agree to its printed precision, with the full q store and with
-`ckpt = 5`. The relaxation run checks the `relaxbox` key.
+`ckpt = 5`. The relaxation run checks the `relaxbox` key. The last test runs the reference's
+shipped input.dat (input.dat:1-11) in its own layout: tabs, trailing comments, E notation and
+keys written against the '='.
 """
 import os
 import re
@@ -106,3 +108,51 @@ def test_relaxbox_key(tmp_path):
     D, g = float(mm.group(2)), float(mm.group(5))
     _say(f"driver relaxed D = {D}, dF/dD = {g}")
     assert abs(g) < 1e-6 and 4.3 < D < 4.7
+
+
+# input.dat:1-11, value for value and in the same layout (tabs, E notation, "readwin=1"),
+# with comments of this file's own.
+SHIPPED_INPUT = (
+    "chi = \t30.00000 \t#chi N\n"
+    "f = \t0.80000 \t#A fraction of the copolymer\n"
+    "ens =\t2\t\t#1 canonical, 2 grand canonical\n"
+    "phidb = 1.16000000E-05 \t#fugacity (GC) or volume fraction (C)\n"
+    "N = \t60 \t\t#contour steps\n"
+    "m = \t200 150 150 \t#grid points\n"
+    "D = \t16.00000 12.00000 12.00000 #box in units of R0\n"
+    "fix = \t0 1 \t\t#end replicas fixed\n"
+    "readwin=1 \t\t#read win files\n"
+    "justFE =1\t\t#free energy only\n"
+    "dostring=1\t\t#string method\n"
+)
+
+
+def test_shipped_input_dat(tmp_path):
+    """The shipped input.dat with no win files: every replica starts from zero fields, W+ is
+    relaxed (solve_field0) and the free energy computed (justFE = 1). Two runtime keys bound
+    the run: strn = 2 (the reference's strn is the compile-time 48, scft.cu:35) and
+    prerelax = 30. Zero fields stay uniform, so the result does not depend on the grid: the
+    oracle runs the same sequence on a 5 x 3 x 3 grid of the same box."""
+    R, prerelax = 2, 30
+    with open(os.path.join(tmp_path, "input.dat"), "w") as fh:
+        fh.write(SHIPPED_INPUT + f"strn = {R}\nprerelax = {prerelax}\n")
+    _say("driver: shipped input.dat, 200 x 150 x 150, FP64")
+    log = _run(tmp_path, timeout=600)
+    assert ("chiN=30 f=0.8 N=60 alpha=0.1 ens=2 phidb=1.16e-05 m=200 150 150 D=16 12 12 strn=2 fix=0 1 "
+            "dostring=1 justFE=1 readwin=1 precision=fp64") in log, log[:2000]
+    assert "win0 is too small" in log and "win1 is too small" in log
+    assert "Solving fields" not in log   # justFE = 1: no string iterations
+    got = np.loadtxt(os.path.join(tmp_path, "FEs"))
+    # identical replicas have zero arc length, so the first column is 0/0 as in scft.cu:846
+    assert got.shape == (R, 3) and np.all(np.isfinite(got[:, 1:])) and np.all(np.isnan(got[:, 0]))
+
+    m, D = (5, 3, 3), (16.0, 12.0, 12.0)
+    g = orc.Grid(m, D)
+    W0 = np.zeros((R, 2, int(np.prod(m))))
+    st = orc.StringState(g, dict(chi=30.0, f=0.8, alpha=0.1, phidb=1.16e-5, N=60, ens=2, dostring=1, fix=(0, 1)), W0)
+    for ii in range(R):
+        g.solve_field0(st.W[ii], st.DEV[ii], 30.0, 0.8, 60, 0.1, 1.16e-5, 2, prerelax, 1e-4)
+    fe = st.free_energy()
+    ref = np.stack([fe["alf"], fe["FEs"], fe["phctot"]], axis=1)
+    _say(f"shipped input.dat: FEs {got[:, 1]} oracle {ref[:, 1]}, phct {got[:, 2]} oracle {ref[:, 2]}")
+    assert np.max(np.abs(got[:, 1:] - ref[:, 1:])) <= 2e-10
