@@ -37,6 +37,9 @@ struct KS {
     static constexpr size_t lds2_plane = (size_t)(PG::LDSN + N1 + N2) * sizeof(cpx<T>);
     static constexpr size_t lds2_col = (size_t)(CG::LDSN + N0) * sizeof(cpx<T>) + (size_t)N0 * sizeof(T);
     static constexpr bool two_ok = PG::ok && CG::ok && lds2_plane <= 160 * 1024 && lds2_col <= 160 * 1024;
+    // planes whose LDS leaves room for one workgroup per CU and that run fewer than 1024
+    // threads (96^2 FP32) use the persistent plane_mid (k_plane_mid2p)
+    static constexpr bool persist_ok = two_ok && lds2_plane > 80 * 1024 && PG::NT < 1024;
     using CGA = C2Geom<N0, TC_ALT>;
     static constexpr size_t lds2_col_alt = (size_t)(CGA::LDSN + N0) * sizeof(cpx<T>) + (size_t)N0 * sizeof(T);
     static constexpr bool alt_ok = two_ok && CGA::ok && lds2_col_alt <= 160 * 1024;
@@ -93,6 +96,11 @@ struct KS {
             e = hipFuncSetAttribute((const void*)k_plane_mid2<T, N1, N2>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds2_plane);
             if (e != hipSuccess) return e;
+            if constexpr (persist_ok) {
+                e = hipFuncSetAttribute((const void*)k_plane_mid2p<T, N1, N2>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2_plane);
+                if (e != hipSuccess) return e;
+            }
             if constexpr (alt_ok) {
                 e = hipFuncSetAttribute((const void*)k_column2<T, N0, N1, N2, TC_ALT>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2_col_alt);
@@ -126,7 +134,17 @@ struct KS {
     {
         if constexpr (two_ok) {
             column_two(a, 2 * a.nb, a.nb, st);
-            hipLaunchKernelGGL((k_plane_mid2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, s);
+            bool persistent = false;
+            if constexpr (persist_ok) {
+                if (a.pgrid > 0) {
+                    const int np = N0 * a.nb;
+                    hipLaunchKernelGGL((k_plane_mid2p<T, N1, N2>), dim3(std::min(a.pgrid, np)), dim3(PG::NT),
+                                       lds2_plane, st, a, s, np);
+                    persistent = true;
+                }
+            }
+            if (!persistent)
+                hipLaunchKernelGGL((k_plane_mid2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, s);
             column_two(a, a.nb, 0, st);
             hipLaunchKernelGGL((k_plane_end2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, s,
                                (int)(s == a.N - 1 ? PL_LAST : PL_STEP));
@@ -379,6 +397,15 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
     ks_ = (pen && pen[0] == '1') ? nullptr : find_kernels<T>(m[0], m[1], m[2]);
     const char* legacy = getenv("SCFT_LEGACY_FFT");
     const bool force_legacy = legacy && legacy[0] == '1';
+    // persistent plane_mid (grids whose kernel set has it): one workgroup per CU;
+    // SCFT_PLANE_GRID sets the count, 0 turns it off
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+            pgrid_ = cus;
+        if (const char* pg = getenv("SCFT_PLANE_GRID")) pgrid_ = std::max(0, atoi(pg));
+    }
     // Without a two-stage set, a Stockham plane that fills a CU's LDS runs one workgroup
     // per CU; the line kernels are faster there (96^3 FP64: 0.529 against 0.805 ms per
     // contour step for 8 replicas).
@@ -659,6 +686,7 @@ PropArgs<T> PropEngine<T>::args(int nb, const ChainConst& cc) const
     a.nb = nb;
     a.pencil_ct = pencil_ct_ ? 1 : 0;
     a.coltile = coltile_;
+    a.pgrid = pgrid_;
     a.Ch = Ch_;
     a.Cf = Cf_;
     a.e4A = e4A_;

@@ -129,6 +129,7 @@ private:
     void (*resume_)(const PropArgs<T>&, int c, hipStream_t) = nullptr;
     int ck_ = 0;      // checkpoint interval (0: every contour index kept)
     int coltile_ = 0;  // two-stage column tile width (0: the grid's default)
+    int pgrid_ = 0;    // workgroups of the persistent plane kernels (0: one per plane)
     int qpitch_ = 0;  // q-store entries per slot
     const char* path_ = "";
     cpx<T>* Ch_ = nullptr;

@@ -188,7 +188,13 @@ struct P2 : P2Geom<NY, NZ> {
     static __device__ __forceinline__ void inv_regs(cpx<T> (&uin)[IY2][B1], cpx<T>* buf, const cpx<T>* twz,
                                                     const cpx<T>* twy, cpx<T> (&zr)[IZ1][A2], int tid)
     {
-        // y stage 2'
+        inv_y2(uin, buf, twy, tid);
+        inv_tail(buf, twz, twy, zr, tid);
+    }
+
+    // first stage of the inverse (y stage 2'): consumes the registers u, result in buf
+    static __device__ __forceinline__ void inv_y2(cpx<T> (&uin)[IY2][B1], cpx<T>* buf, const cpx<T>* twy, int tid)
+    {
 #pragma unroll
         for (int it = 0; it < IY2; ++it) {
             const int w = tid + it * NT;
@@ -203,6 +209,12 @@ struct P2 : P2Geom<NY, NZ> {
                 });
             }
         }
+    }
+
+    // remaining stages of the inverse, from buf (after inv_y2) to Z1-layout zr
+    static __device__ __forceinline__ void inv_tail(cpx<T>* buf, const cpx<T>* twz, const cpx<T>* twy,
+                                                    cpx<T> (&zr)[IZ1][A2], int tid)
+    {
         __syncthreads();
         // y stage 1'
         cpx<T> yy[IY1][B2];
@@ -440,6 +452,68 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT
         }
     __syncthreads();
     P::fwd(z, buf, twz, twy, a.Ch + fo, tid);
+}
+
+// Persistent form of plane_mid for planes that take one workgroup per CU (their LDS). A
+// workgroup with its CU to itself cannot overlap one plane's global loads with another
+// plane's transforms, so each workgroup here walks the planes w = blockIdx.x,
+// blockIdx.x + gridDim.x, ... of the batch (plane w of slot w / m0 at offset w NE) and loads
+// the next plane's k-space input during the current plane's forward transform. Barriers wait
+// for LDS only, so those loads stay in flight through it. Requires gridDim.x <= nplanes =
+// m0 * nb. Measured on MI355X: 96^3 FP32 x 8, 0.1697 -> 0.1632 ms per contour step. The same
+// for plane_end, and for 128^2 planes (1024 threads, at most 128 VGPRs), spills registers and
+// was slower (128^3 FP32 x 4: 0.21 -> 0.26 ms with mid, 0.41 with both).
+template <typename T, int NY, int NZ>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT)))
+k_plane_mid2p(PropArgs<T> a, int s, int nplanes)
+{
+    using P = P2<T, NY, NZ>;
+    constexpr int NT = P::NT, IZ1 = P::IZ1, A2 = P::A2, NE = NY * NZ;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* twz = buf + P::LDSN;
+    cpx<T>* twy = twz + NZ;
+    const int tid0 = threadIdx.x, G = gridDim.x;
+    p2_load_tw<T, NY, NZ>(twz, twy, a, tid0, NT);
+    const bool qa = s < a.n1, pb = s < a.n2;
+    cpx<T> u[P::IY2][P::B1];
+    P::inv_load(a.Ch + (size_t)blockIdx.x * NE, u, tid0);
+    __syncthreads();  // twiddles
+    for (int w = blockIdx.x; w < nplanes; w += G) {
+        const size_t fo = (size_t)w * NE;
+        // per-thread offsets are recomputed in every iteration rather than held in registers
+        // across the loop
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        P::inv_y2(u, buf, twy, tid);
+        T ex[IZ1][A2], ey[IZ1][A2];
+#pragma unroll
+        for (int it = 0; it < IZ1; ++it) {
+            const int wi = tid + it * NT;
+            if (P::z1_valid(wi)) {
+#pragma unroll
+                for (int a2 = 0; a2 < A2; ++a2) {
+                    const int e = P::z1_elem(wi, a2);
+                    const T va = a.e4A[fo + e], vb = a.e4B[fo + e];
+                    ex[it][a2] = qa ? va : vb;
+                    ey[it][a2] = pb ? vb : va;
+                }
+            }
+        }
+        cpx<T> z[IZ1][A2];
+        P::inv_tail(buf, twz, twy, z, tid);
+#pragma unroll
+        for (int it = 0; it < IZ1; ++it)
+#pragma unroll
+            for (int a2 = 0; a2 < A2; ++a2) {
+                z[it][a2].x = Boltz<T>::mul2(z[it][a2].x, ex[it][a2]);
+                z[it][a2].y = Boltz<T>::mul2(z[it][a2].y, ey[it][a2]);
+            }
+        if (w + G < nplanes) P::inv_load(a.Ch + fo + (size_t)G * NE, u, tid);
+        __syncthreads();
+        P::fwd(z, buf, twz, twy, a.Ch + fo, tid);
+        __syncthreads();  // buf is reused by the next plane
+    }
 }
 
 // Column pass on the two-stage FFT: for TC consecutive (y,z) columns of one field, forward

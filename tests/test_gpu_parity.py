@@ -174,6 +174,7 @@ LARGE = [
     ((48, 48, 48), (5.0, 5.0, 5.0), 12, 0.5, 0.1, 1, 0.8, "fp64", TOL64),
     ((64, 64, 64), (6.0, 6.0, 6.0), 16, 0.45, 0.125, 1, 0.9, "fp64", TOL64),
     ((96, 96, 96), (8.0, 8.0, 8.0), 8, 0.5, 0.125, 2, 0.7, "fp64", TOL64),
+    ((96, 96, 96), (8.0, 8.0, 8.0), 8, 0.5, 0.125, 1, 0.8, "fp32", 2e-5),
     ((128, 128, 128), (8.0, 8.0, 8.0), 8, 0.5, 0.125, 1, 0.8, "fp32", 2e-5),
 ]
 
@@ -190,6 +191,33 @@ def test_props_large_grids_vs_oracle(case):
     dphi = max(_rel(phi[0, c], ref["phi"][c]) for c in range(4))
     _say(f"  {prec} {m}: rel dlnQ={dl:.2e} rel dphi={dphi:.2e}")
     assert dl <= tol and dphi <= tol
+
+
+@pytest.mark.parametrize("pgrid", ["7", "256"])
+def test_persistent_plane_mid_matches_per_plane(pgrid, monkeypatch):
+    """96^2 FP32 planes run plane_mid as a persistent kernel: SCFT_PLANE_GRID workgroups (by
+    default one per CU) each walk planes w, w + grid, ... and load the next plane during the
+    current one's forward transform. SCFT_PLANE_GRID=0 runs one workgroup per plane. Both run
+    the same transforms; FP32 rounding may differ where the compiler contracts differently.
+    Three slots give 288 planes, so 7 workgroups take 41-42 planes each, and 256 take one or
+    two."""
+    m, D = (96, 96, 96), (6.0, 6.0, 6.0)
+    W = np.stack([_fields(m, D, seed=70 + i, amp=1.5) for i in range(3)])
+    out = {}
+    for pg in ("0", pgrid):
+        monkeypatch.setenv("SCFT_PLANE_GRID", pg)
+        with Context(m, D, 24, 0.45, 0.1, 1, 0.9, chi=18.0, nrep=3, precision="fp32") as ctx:
+            assert ctx.fft_path == "two-stage"
+            phi, lnQ, _ = ctx.props_batch(W)
+            qq = ctx.qq_invariant(2)
+        out[pg] = (phi, lnQ, qq)
+    dl = float(np.max(np.abs(out["0"][1] - out[pgrid][1])))
+    dphi = _rel(out[pgrid][0], out["0"][0])
+    qq = out[pgrid][2]
+    dev = float(np.max(np.abs(qq - qq[0])) / abs(qq[0]))
+    _say(f"persistent plane_mid (grid {pgrid}) vs per plane: |dlnQ| {dl:.2e}, rel dphi {dphi:.2e}, "
+         f"invariant {dev:.2e}")
+    assert dl <= 2e-5 and dphi <= 2e-5 and dev <= 1e-5
 
 
 def test_96cube_fp64_stockham_and_line_kernels_agree(monkeypatch):
