@@ -27,10 +27,12 @@ struct P2Geom {
     static constexpr int NT = cmax(64, cdiv(NY * A1, 64) * 64) > 1024 ? 1024 : cmax(64, cdiv(NY * A1, 64) * 64);
     static constexpr int IZ1 = cdiv(NY * A1, NT), IZ2 = cdiv(NY * A2, NT);
     static constexpr int IY1 = cdiv(NZ * B1, NT), IY2 = cdiv(NZ * B2, NT);
-    // Wave-local z transposes: with A1 == A2 the lanes owning row y in the z stages are the
-    // same A1 consecutive lanes of one wave in both stages, and the TZ and PL row regions
-    // coincide, so the z-stage exchanges need no workgroup barrier.
-    static constexpr bool WLZ = A1 == A2 && 64 % A1 == 0;
+    // Wave-local z transposes: when A1 divides A2 and 64, each stage-1 item (y, a1) also
+    // takes the stage-2 items (y, a1 + A1 j), j < A2 / A1 (z2_item). The lanes owning row y
+    // are then the same A1 consecutive lanes of one wave in both stages, and the TZ and PL row
+    // regions coincide, so the z-stage exchanges need no workgroup barrier.
+    static constexpr bool WLZ = A2 % A1 == 0 && 64 % A1 == 0;
+    static constexpr int IZ2E = WLZ ? IZ1 * (A2 / A1) : IZ2;  // stage-2 items per thread
     // LDS layouts in complex elements
     static constexpr int PZ = A2 + 1;                                        // TZ: y*RZ + a1*PZ + k2
     static constexpr int RZ = WLZ ? cmax(A1 * PZ, NZ + 8) : A1 * PZ + 4;
@@ -45,7 +47,24 @@ struct P2 : P2Geom<NY, NZ> {
     using G = P2Geom<NY, NZ>;
     using G::A1; using G::A2; using G::B1; using G::B2; using G::NT;
     using G::IZ1; using G::IZ2; using G::IY1; using G::IY2;
-    using G::PZ; using G::RZ; using G::PP; using G::PY; using G::CY; using G::WLZ;
+    using G::PZ; using G::RZ; using G::PP; using G::PY; using G::CY; using G::WLZ; using G::IZ2E;
+
+    // z stage-2 item `it` of thread tid: its row y and k2, false when it has none
+    static __device__ __forceinline__ bool z2_item(int tid, int it, int& y, int& k2)
+    {
+        if constexpr (WLZ) {
+            constexpr int R = A2 / A1;
+            const int w = tid + (it / R) * NT;
+            y = w / A1;
+            k2 = w % A1 + A1 * (it % R);
+            return (NY * A1) % NT == 0 || w < NY * A1;
+        } else {
+            const int w = tid + it * NT;
+            y = w / A2;
+            k2 = w % A2;
+            return (NY * A2) % NT == 0 || w < NY * A2;
+        }
+    }
 
     // exchange point inside the z stages: a wave-level ordering point when the exchange
     // stays inside one wave, else a workgroup barrier
@@ -81,12 +100,11 @@ struct P2 : P2Geom<NY, NZ> {
         }
         zsync();
         // z stage 2
-        cpx<T> zz[IZ2][A1];
+        cpx<T> zz[IZ2E][A1];
 #pragma unroll
-        for (int it = 0; it < IZ2; ++it) {
-            const int w = tid + it * NT;
-            if ((NY * A2) % NT == 0 || w < NY * A2) {
-                const int y = w / A2, k2 = w % A2;
+        for (int it = 0; it < IZ2E; ++it) {
+            int y, k2;
+            if (z2_item(tid, it, y, k2)) {
                 sfor<A1>([&](auto ac) {
                     constexpr int a1 = decltype(ac)::value;
                     zz[it][a1] = buf[y * RZ + a1 * PZ + k2];
@@ -96,10 +114,9 @@ struct P2 : P2Geom<NY, NZ> {
         }
         zsync();
 #pragma unroll
-        for (int it = 0; it < IZ2; ++it) {
-            const int w = tid + it * NT;
-            if ((NY * A2) % NT == 0 || w < NY * A2) {
-                const int y = w / A2, k2 = w % A2;
+        for (int it = 0; it < IZ2E; ++it) {
+            int y, k2;
+            if (z2_item(tid, it, y, k2)) {
                 sfor<A1>([&](auto kc) {
                     constexpr int k1 = decltype(kc)::value;
                     buf[y * PP + k2 + A2 * k1] = zz[it][k1];
@@ -244,12 +261,11 @@ struct P2 : P2Geom<NY, NZ> {
         }
         __syncthreads();
         // z stage 2'
-        cpx<T> zz[IZ2][A1];
+        cpx<T> zz[IZ2E][A1];
 #pragma unroll
-        for (int it = 0; it < IZ2; ++it) {
-            const int w = tid + it * NT;
-            if ((NY * A2) % NT == 0 || w < NY * A2) {
-                const int y = w / A2, k2 = w % A2;
+        for (int it = 0; it < IZ2E; ++it) {
+            int y, k2;
+            if (z2_item(tid, it, y, k2)) {
                 sfor<A1>([&](auto kc) {
                     constexpr int k1 = decltype(kc)::value;
                     zz[it][k1] = buf[y * PP + k2 + A2 * k1];
@@ -263,10 +279,9 @@ struct P2 : P2Geom<NY, NZ> {
         }
         zsync();
 #pragma unroll
-        for (int it = 0; it < IZ2; ++it) {
-            const int w = tid + it * NT;
-            if ((NY * A2) % NT == 0 || w < NY * A2) {
-                const int y = w / A2, k2 = w % A2;
+        for (int it = 0; it < IZ2E; ++it) {
+            int y, k2;
+            if (z2_item(tid, it, y, k2)) {
                 sfor<A1>([&](auto ac) {
                     constexpr int a1 = decltype(ac)::value;
                     buf[y * RZ + a1 * PZ + k2] = zz[it][a1];
