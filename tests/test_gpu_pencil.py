@@ -112,7 +112,7 @@ def test_pencil_symmetric_operator_invariant(prec, tol):
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
 def test_pencil_and_two_stage_agree(prec, tol, monkeypatch):
-    # 48 = 6 x 8 in the two-stage kernels and 8 x 2 x 3 in the line kernels. At 64 both run
+    # 48 = 4 x 12 (z) and 6 x 8 (y) in the two-stage kernels, 8 x 2 x 3 in the line kernels. At 64 both run
     # the same radix-8 four-step arithmetic, and FP32 agrees bit for bit.
     m, D = (48, 48, 48), (5.0, 5.0, 5.0)
     W = np.stack([_fields(m, D, seed=40 + i) for i in range(3)])
