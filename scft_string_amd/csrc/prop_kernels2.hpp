@@ -60,7 +60,7 @@ template <typename T, int NY, int NZ>
 struct P2 : P2Geom<NY, NZ> {
     using G = P2Geom<NY, NZ>;
     using G::A1; using G::A2; using G::B1; using G::B2; using G::NT;
-    using G::IZ1; using G::IZ2; using G::IY1; using G::IY2;
+    using G::IZ1; using G::IY1; using G::IY2;
     using G::PZ; using G::RZ; using G::PP; using G::PY; using G::CY; using G::WLZ; using G::IZ2E;
 
     // z stage-2 item `it` of thread tid: its row y and k2, false when it has none
