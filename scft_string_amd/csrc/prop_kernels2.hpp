@@ -345,10 +345,13 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT
     const size_t fo = (size_t)slot * (size_t)a.M + (size_t)x * NE;
     p2_load_tw<T, NY, NZ>(twz, twy, a, tid, NT);
 
-    // Up to 64^2 planes every global load of the plane is issued up front (both k-space
-    // inputs and the factors). Larger planes run 1024 threads at <= 128 VGPRs, so they
-    // load Cf after the Ch inverse and the factors after both inverses instead of spilling.
-    constexpr bool EARLY = NE <= 4096;
+    // Below 64^2 every global load of the plane is issued up front (both k-space inputs and
+    // the factors). From 64^2 up the kernel loads Cf after the Ch inverse and the factors
+    // after both inverses: larger planes would spill otherwise, and at 64^2 the fewer live
+    // registers (72 against 89 VGPRs in FP32) fit a third workgroup per CU. Measured per
+    // contour step at 14 x 64^3: FP64 0.1551 -> 0.1456 ms, FP32 0.0768 -> 0.0760. At 32^2
+    // late loads cost 12%.
+    constexpr bool EARLY = NE < 4096;
     cpx<T> uh[P::IY2][P::B1], uf[P::IY2][P::B1];
     if (mode != PL_BEGIN && mode != PL_RESUME) {
         P::inv_load(a.Ch + fo, uh, tid);
@@ -455,23 +458,29 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT
     p2_load_tw<T, NY, NZ>(twz, twy, a, tid, NT);
     const bool qa = s < a.n1, pb = s < a.n2;
     T ex[IZ1][A2], ey[IZ1][A2];
+    auto load_factors = [&]() {
 #pragma unroll
-    for (int it = 0; it < IZ1; ++it) {
-        const int w = tid + it * NT;
-        if (P::z1_valid(w)) {
+        for (int it = 0; it < IZ1; ++it) {
+            const int w = tid + it * NT;
+            if (P::z1_valid(w)) {
 #pragma unroll
-            for (int a2 = 0; a2 < A2; ++a2) {
-                const int e = P::z1_elem(w, a2);
-                const T va = a.e4A[fo + e], vb = a.e4B[fo + e];
-                const T x4 = qa ? va : vb, y4 = pb ? vb : va;
-                ex[it][a2] = x4;
-                ey[it][a2] = y4;
+                for (int a2 = 0; a2 < A2; ++a2) {
+                    const int e = P::z1_elem(w, a2);
+                    const T va = a.e4A[fo + e], vb = a.e4B[fo + e];
+                    ex[it][a2] = qa ? va : vb;
+                    ey[it][a2] = pb ? vb : va;
+                }
             }
         }
-    }
+    };
+    // factors with the k-space input below 64^2, after the inverse from 64^2 up (as in
+    // k_plane_end2: 71 against 91 VGPRs at 64^2 FP32)
+    constexpr bool EARLY = NE < 4096;
+    if constexpr (EARLY) load_factors();
     __syncthreads();  // twiddles
     cpx<T> z[IZ1][A2];
     P::inv(a.Ch + fo, buf, twz, twy, z, tid);
+    if constexpr (!EARLY) load_factors();
 #pragma unroll
     for (int it = 0; it < IZ1; ++it)
 #pragma unroll
