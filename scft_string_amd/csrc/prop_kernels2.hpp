@@ -330,9 +330,18 @@ __device__ __forceinline__ void p2_load_tw(cpx<T>* twz, cpx<T>* twy, const PropA
     for (int i = tid; i < NY; i += nt) twy[i] = a.tw1[i];
 }
 
+// Minimum waves per SIMD k_plane_end2 is compiled for (1: the compiler's choice). The FP64
+// 64^2 plane otherwise takes 132 VGPRs, one 512-thread workgroup per CU; capped at 4 waves it
+// takes 126 with no spill and two workgroups fit (64^3 FP64 x 14: 0.1456 -> 0.1367 ms per
+// contour step, bit-identical results).
+template <typename T, int NE>
+constexpr int end_wpe() { return (NE == 4096 && sizeof(T) == 8) ? 4 : 1; }
+
 // plane_end on the two-stage FFT (see k_plane_end in prop_kernels.hpp for the contract)
 template <typename T, int NY, int NZ>
-__global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT))) k_plane_end2(PropArgs<T> a, int s, int mode)
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT),
+                               amdgpu_waves_per_eu(end_wpe<T, NY * NZ>())))
+k_plane_end2(PropArgs<T> a, int s, int mode)
 {
     using P = P2<T, NY, NZ>;
     constexpr int NT = P::NT, IZ1 = P::IZ1, A2 = P::A2, NE = NY * NZ;
