@@ -10,10 +10,16 @@ divided by N), the kernel family, and the algorithmic HBM rate against the 8 TB/
 (SCFT_PENCIL=1); --runtime-lengths forces the runtime-planned line kernels
 (SCFT_PENCIL_RT=1). --fields lamella times a lamellar field state instead of W = 0: the
 step time depends on the data, and zero fields run measurably faster.
+
+    python -m scft_string_amd.timing --stream-probe
+
+prints the streaming rates of torch's own copy and add kernels (256 MiB to 4 GiB), a
+practical ceiling to set the contour step's measured HBM traffic against.
 """
 import argparse
 import json
 import os
+import sys
 
 import numpy as np
 
@@ -38,7 +44,38 @@ def lamella_fields(m, D, R, amp=1.5, seed=0):
     return W
 
 
+def stream_probe(sizes_mib=(256, 1024, 4096), iters=20):
+    """Streaming rates of torch's own elementwise kernels on this GPU, as a ceiling that
+    kernels on it can reach in practice: copy (one read, one write) and add (two reads, one
+    write) of FP32 arrays, timed with HIP events. Prints one JSON line per kernel and size."""
+    import torch
+
+    for mib in sizes_mib:
+        n = (mib << 20) // 4
+        a = torch.ones(n, device="cuda")
+        b = torch.full((n,), 2.0, device="cuda")
+        c = torch.empty(n, device="cuda")
+        for name, fn, moved in (("copy", lambda: c.copy_(a), 2 * (mib << 20)),
+                                ("add", lambda: torch.add(a, b, out=c), 3 * (mib << 20))):
+            for _ in range(3):
+                fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(iters):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            t = e0.elapsed_time(e1) / iters * 1e-3
+            print(json.dumps({"kernel": name, "MiB": mib, "us": round(t * 1e6, 1),
+                              "GBps": round(moved / t / 1e9, 1)}), flush=True)
+        del a, b, c
+
+
 def main(argv=None):
+    if "--stream-probe" in (sys.argv[1:] if argv is None else argv):
+        stream_probe()
+        return
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--m", type=int, nargs=3, required=True)
     ap.add_argument("--D", type=float, nargs=3, required=True)
