@@ -37,7 +37,7 @@ struct KS {
     static constexpr size_t lds2_plane = (size_t)(PG::LDSN + N1 + N2) * sizeof(cpx<T>);
     static constexpr size_t lds2_col = (size_t)(CG::LDSN + N0) * sizeof(cpx<T>) + (size_t)N0 * sizeof(T);
     static constexpr bool two_ok = PG::ok && CG::ok && lds2_plane <= 160 * 1024 && lds2_col <= 160 * 1024;
-    // planes whose LDS leaves room for one workgroup per CU and that run fewer than 1024
+    // planes whose LDS allows only one workgroup per CU and that run fewer than 1024
     // threads (96^2 FP32) use the persistent plane_mid (k_plane_mid2p)
     static constexpr bool persist_ok = two_ok && lds2_plane > 80 * 1024 && PG::NT < 1024;
     using CGA = C2Geom<N0, TC_ALT>;

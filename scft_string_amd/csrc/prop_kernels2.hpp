@@ -15,11 +15,6 @@ namespace scft {
 constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
-// Plane geometry: rows y (NY), columns z (NZ). Work items per stage:
-//   Z1 (y, a1)  holds x[y][a1 + A1 a2], a2 < A2      (real-space ownership)
-//   Z2 (y, k2)  holds Z[y][k2 + A2 k1], k1 < A1      (after the z transform)
-//   Y1 (kz, b1) holds P[b1 + B1 b2][kz], b2 < B2
-//   Y2 (kz, l2) holds Y[l2 + B2 l1][kz], l1 < B1     (k-space ownership, global layout)
 // z split of a plane: from length 32 up, one that allows wave-local z exchanges (N1 divides
 // N2 and 64) when the length has one, else split_n1. This changes 48 from 6 x 8 to 4 x 12:
 // measured per contour step, 48^3 FP64 x 16 0.0925 -> 0.0802 ms, FP32 x 1 0.0294 -> 0.0281,
@@ -34,6 +29,11 @@ constexpr int split_wlz(int N)
     return split_n1(N);
 }
 
+// Plane geometry: rows y (NY), columns z (NZ). Work items per stage:
+//   Z1 (y, a1)  holds x[y][a1 + A1 a2], a2 < A2      (real-space ownership)
+//   Z2 (y, k2)  holds Z[y][k2 + A2 k1], k1 < A1      (after the z transform)
+//   Y1 (kz, b1) holds P[b1 + B1 b2][kz], b2 < B2
+//   Y2 (kz, l2) holds Y[l2 + B2 l1][kz], l1 < B1     (k-space ownership, global layout)
 template <int NY, int NZ>
 struct P2Geom {
     static constexpr int A1 = split_wlz(NZ), A2 = NZ / A1;

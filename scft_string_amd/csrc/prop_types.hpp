@@ -14,8 +14,8 @@ struct PropArgs {
     int nb;                // active batch slots
     int pencil_ct;         // line kernels: use the compile-time-length ones where they exist
     int coltile;           // two-stage column pass: tile width (0: the grid's default)
-    int pgrid;             // two-stage plane kernels of one-workgroup-per-CU planes: > 0 runs
-                           // the persistent form on that many workgroups; 0 one per plane
+    int pgrid;             // planes with a persistent plane_mid: > 0 runs it on that many
+                           // workgroups; 0 launches one workgroup per plane
     cpx<T>* Ch;            // [slot][M] half-step sandwich work field
     cpx<T>* Cf;            // [slot][M] full-step sandwich work field
     const T* e4A;          // [slot][M] quarter-step factor of species A (Boltz<T> form)
