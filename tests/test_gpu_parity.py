@@ -77,18 +77,25 @@ def test_props_fp32_error_vs_oracle(m):
     assert dphi < 1e-4
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
-def test_symmetric_operator_invariant(prec, tol):
+@pytest.mark.parametrize("m,prec,tol", [
+    ((32, 32, 32), "fp64", 1e-12), ((32, 32, 32), "fp32", 2e-5),
+    ((48, 48, 48), "fp64", 1e-12),      # z split 4 x 12 (wave-local z exchanges)
+    ((64, 64, 64), "fp64", 1e-12),      # plane_end compiled for two workgroups per CU
+    ((64, 64, 64), "fp32", 2e-5),
+    ((128, 128, 128), "fp32", 2e-5),    # z split 8 x 16, two stage-2 items per lane
+], ids=lambda v: "x".join(map(str, v)) if isinstance(v, tuple) else str(v))
+def test_symmetric_operator_invariant(m, prec, tol):
     """<q1[s] q2[N-s]> is independent of s (fails for a wrong species switch, factor
-    order or Richardson weight)."""
-    m, D = (32, 32, 32), (4.0, 4.0, 4.0)
+    order or Richardson weight), on each plane-kernel geometry."""
+    D = tuple(v / 8.0 for v in m)
     N = 48
     W = _fields(m, D, seed=5, amp=3.0)[None]
     with Context(m, D, N, 0.4, 0.1, 1, 0.8, chi=15.0, nrep=1, precision=prec) as ctx:
+        assert ctx.fft_path == "two-stage"
         ctx.props_batch(W)
         qq = ctx.qq_invariant(0)
     spread = (qq.max() - qq.min()) / qq.mean()
-    _say(f"{prec}: qq spread {spread:.3e}")
+    _say(f"{m} {prec}: qq spread {spread:.3e}")
     assert spread < tol
 
 
