@@ -52,6 +52,9 @@ CASES = [
     ((20, 15, 15), (2.0, 1.5, 1.5), 24, 0.55, 0.4, 2, 0.5, 0.3, "fp32", 2, 4, {}),
     ((12, 8, 16), (3.0, 2.5, 4.0), 21, 0.4, 0.1, 1, 0.7, 0.0, "fp64", 2, 1, {}),
     ((12, 8, 16), (3.0, 2.5, 4.0), 21, 0.4, 0.1, 1, 0.7, 0.0, "fp64", 2, 30, {}),
+    # persistent plane_mid (96^2 FP32) and the 4 x 12 z split of 48 inside the recompute windows
+    ((96, 96, 96), (6.0, 6.0, 6.0), 24, 0.45, 0.1, 1, 0.9, 0.0, "fp32", 3, 5, {}),
+    ((48, 48, 48), (4.0, 4.0, 4.0), 30, 0.5, 0.1, 2, 0.6, 0.0, "fp64", 2, 6, {}),
 ]
 
 
