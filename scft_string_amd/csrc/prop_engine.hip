@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 namespace scft {
@@ -160,7 +161,7 @@ struct KS {
     {
         KernelSet<T> k{N0,      N1,      N2,      &setup_legacy, &begin_legacy,  &step_legacy,
                        nullptr, nullptr, nullptr, "stockham",    &resume_legacy, nullptr,
-                       lds_plane > 80 * 1024, TC, alt_ok ? TC_ALT : 0};
+                       lds_plane > 80 * 1024, TC, alt_ok ? TC_ALT : 0, persist_ok};
         if (two_ok) {
             k.setup2 = &setup_two;
             k.begin2 = &begin_two;
@@ -564,9 +565,11 @@ hipError_t PropEngine<T>::tune(const PropArgs<T>& a, int nb, const ChainConst& c
 template <typename T>
 std::string PropEngine<T>::config() const
 {
-    char b[224];
+    char b[320];
     int n = snprintf(b, sizeof b, "%s, %d replica-group stream%s, %s%s", path_, nstreams_, nstreams_ > 1 ? "s" : "",
                      use_graph_ ? "graph replay" : "direct launches", tuned_ ? " (autotuned)" : "");
+    if (ks_ && ks_->persist_mid && pgrid_ > 0 && strncmp(path_, "two-stage", 9) == 0 && n > 0 && n < (int)sizeof b)
+        n += snprintf(b + n, sizeof b - n, ", persistent plane_mid on %d workgroups", pgrid_);
     if (ck_ && n > 0 && n < (int)sizeof b)
         snprintf(b + n, sizeof b - n, ", q checkpointed every %d steps (%d of %d entries per replica)", ck_, qpitch_,
                  N_ + 1);

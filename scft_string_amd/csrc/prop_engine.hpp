@@ -28,6 +28,8 @@ struct KernelSet {
     bool big_plane;
     // two-stage column tile: the default width and the alternative (0: none)
     int tc, tc_alt;
+    // the two-stage set has the persistent plane_mid (k_plane_mid2p)
+    bool persist_mid;
 };
 
 // kernel set for an (m0, m1, m2) grid, or nullptr when no instantiation exists

@@ -215,6 +215,7 @@ def test_persistent_plane_mid_matches_per_plane(pgrid, monkeypatch):
         monkeypatch.setenv("SCFT_PLANE_GRID", pg)
         with Context(m, D, 24, 0.45, 0.1, 1, 0.9, chi=18.0, nrep=3, precision="fp32") as ctx:
             assert ctx.fft_path == "two-stage"
+            assert ("persistent plane_mid" in ctx.engine_info) == (pg != "0"), ctx.engine_info
             phi, lnQ, _ = ctx.props_batch(W)
             qq = ctx.qq_invariant(2)
         out[pg] = (phi, lnQ, qq)
