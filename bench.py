@@ -131,6 +131,49 @@ def precision_report(W, Wshape_m, c, oracle_rows=(0,)):
     return out
 
 
+def trajectory_report(iters, control=False):
+    """The benchmark string run independently in FP32 and in FP64 from the same seeded fields
+    for `iters` string iterations (FreeE at k % 50 == 0 and at the end, as the reference's
+    driver calls it, scft.cu:766-779, 1132), then per-replica |dF| of the final FreeE, max
+    |d phiA| and the last iteration's error norms of the two runs. With `control`, the first
+    run is FP64 too, from the same fields plus 1e-12 N(0, 1) on the interior replicas' W-:
+    how far the iteration itself carries a perturbation far below FP32 roundoff."""
+    from scft_string_amd import Context
+
+    c = CFG
+    R = c["nrep"]
+    res = {}
+    W0 = initial_fields(c["m"], c["D"], R)
+    for run, prec in (("fp32", "fp64" if control else "fp32"), ("fp64", "fp64")):
+        W = W0.copy()
+        if control and run == "fp32":
+            W[1:-1, 0] += 1e-12 * np.random.default_rng(7).standard_normal(W[1:-1, 0].shape)
+        with Context(c["m"], c["D"], c["N"], c["f"], c["alpha"], c["ens"], c["phidb"], c["chi"], nrep=R,
+                     precision=prec, fix=c["fix"]) as ctx:
+            ctx.set_fields(W)
+            t0 = time.time()
+            rep = None
+            for k in range(iters):
+                rep = ctx.string_iterate(k)
+                if k % 50 == 0:
+                    ctx.free_energy()
+                if k % 20 == 0:
+                    print(f"trajectory {run} ({prec}): iteration {k}", file=sys.stderr, flush=True)
+            fe = ctx.free_energy()["FEs"]
+            phi = ctx.get_densities()
+            res[run] = (fe, phi[:, 0].copy(), np.asarray(rep["errs3"]), time.time() - t0)
+    d = np.abs(res["fp32"][0] - res["fp64"][0])
+    rel = (res["fp32"][0] - res["fp32"][0][0]) - (res["fp64"][0] - res["fp64"][0][0])
+    what = ("FP64 and FP64 from fields perturbed by 1e-12" if control else "independent FP32 and FP64 runs")
+    return {"mode": "trajectory", "workload": f"benchmark string, 16 x 64^3, N=128, {what}",
+            "iterations": iters,
+            "max_abs_dF_per_replica": float(d.max()), "max_abs_dF_between_replicas": float(np.abs(rel).max()),
+            "abs_dF": [float(v) for v in d], "max_abs_dphiA": float(np.max(np.abs(res["fp32"][1] - res["fp64"][1]))),
+            "errs3_first_run_last": float(res["fp32"][2].max()), "errs3_fp64_last": float(res["fp64"][2].max()),
+            "FEs_fp64": [float(v) for v in res["fp64"][0]],
+            "wall_s": {"fp32": round(res["fp32"][3], 1), "fp64": round(res["fp64"][3], 1)}}
+
+
 def plain_fp64_rate(W, c, props_per_iter, reps=2):
     """The plain per-replica FP64 GPU path (hipFFT, unfused, blocking host copies per props;
     scft_string_amd/plain.py): time `reps` props of one replica and scale to the string
@@ -340,6 +383,10 @@ def main():
                     help="box-gradient timing: 64^3, N=100, FP32, D swept over 5.6-6.4, 1 and 8 replicas")
     ap.add_argument("--uneven", action="store_true",
                     help="24 x 128^3 N=160 string + 12 independent solves, concurrent vs sequential")
+    ap.add_argument("--trajectory", type=int, default=0, metavar="ITERS",
+                    help="independent FP32 and FP64 runs of the benchmark string for ITERS iterations: |dF|, |d phiA|")
+    ap.add_argument("--trajectory-control", action="store_true",
+                    help="with --trajectory: FP64 against FP64 from fields perturbed by 1e-12")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: all string replicas share it; "
@@ -354,6 +401,9 @@ def main():
         return
     if a.uneven:
         uneven_report(a.steps)
+        return
+    if a.trajectory:
+        print(json.dumps(trajectory_report(a.trajectory, a.trajectory_control)), flush=True)
         return
 
     import torch
