@@ -10,6 +10,14 @@ interpolants plus N(0, 0.01^2) noise (seed 2024); W+ = 0. A step is one solve_fi
 perpendicular projection, mixing, arc length, spline redistribution, with FreeE at k % 50 == 0.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Other modes, each printing one JSON line of its own:
+
+    python bench.py --adjoint            box-gradient cost, full and checkpointed q store
+    python bench.py --uneven             uneven batch: 128^3 string + 12 solves, concurrent
+    python bench.py --trajectory ITERS   independent FP32 and FP64 runs of this string
+                    [--trajectory-control]  (FP64 against FP64 from fields perturbed by 1e-12)
+    python bench.py --pmc-probe          a 512 MiB copy, then one props: the rocprofv3 --pmc run
 """
 import argparse
 import json
