@@ -511,6 +511,19 @@ __global__ void __launch_bounds__(RED_NT) k_density(const cpx<T>* __restrict__ q
     double* ph = phi + (size_t)slot2rep[slot] * (size_t)phistride;
     double acc[5] = {0, 0, 0, 0, 0};
     for (long long r = (long long)blockIdx.x * RED_NT + threadIdx.x; r < M; r += (long long)gridDim.x * RED_NT) {
+        // homopolymers: B = 2 q2[Nh] + sum_{s=1}^{Nh-1} w_s q2[s] q2[Nh-s] (density.h:409-412),
+        // A the same over q1 (density.h:404-407), from the same slots' real parts. First, with
+        // ordinary loads, so that the copolymer loop below finds slots 1..Nh in the cache
+        const cpx<T> zh = q[(size_t)Nh * M + r];
+        double hB = 2.0 * (double)zh.y, hA = 2.0 * (double)zh.x;
+        for (int s = 1; 2 * s <= Nh; ++s) {
+            const int t = Nh - s;
+            if (t < 1) break;
+            const cpx<T> zs = q[(size_t)s * M + r], zt = q[(size_t)t * M + r];
+            const double w = (t != s) ? wt_odd_first(s) + wt_odd_first(t) : wt_odd_first(s);
+            hB += w * ((double)zs.y * (double)zt.y);
+            hA += w * ((double)zs.x * (double)zt.x);
+        }
         double cA = 0, cB = 0;
         // read in pairs (s, N-s), DU pairs per batch: every load of a batch is in flight
         // before the first product (the slots are M apart, so nothing else orders them)
@@ -550,18 +563,6 @@ __global__ void __launch_bounds__(RED_NT) k_density(const cpx<T>* __restrict__ q
                 cA += w2.a * v2;
                 cB += w2.b * v2;
             }
-        }
-        // homopolymers: B = 2 q2[Nh] + sum_{s=1}^{Nh-1} w_s q2[s] q2[Nh-s] (density.h:409-412),
-        // A the same over q1 (density.h:404-407), from the same slots' real parts
-        const cpx<T> zh = load_q(q + (size_t)Nh * M + r);
-        double hB = 2.0 * (double)zh.y, hA = 2.0 * (double)zh.x;
-        for (int s = 1; 2 * s <= Nh; ++s) {
-            const int t = Nh - s;
-            if (t < 1) break;
-            const cpx<T> zs = load_q(q + (size_t)s * M + r), zt = load_q(q + (size_t)t * M + r);
-            const double w = (t != s) ? wt_odd_first(s) + wt_odd_first(t) : wt_odd_first(s);
-            hB += w * ((double)zs.y * (double)zt.y);
-            hA += w * ((double)zs.x * (double)zt.x);
         }
         density_point(cA, cB, hA, hB, sC, sH, sHA, ph, r, M, acc);
     }
