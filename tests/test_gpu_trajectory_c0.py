@@ -7,7 +7,10 @@ The parity tests teacher-force single iterations from identical states. Here bot
 40 iterations from the same start without being reset, so any difference that compounds
 over iterations shows: the error norm of every iteration (relative 1e-9), the number of
 props each iteration runs (equal), and the fields and free energy at the end (1e-9).
+SCFT_C0_ITERS=200 runs the configuration's full 200 iterations (about 3 minutes, most of it
+the oracle on 16 threads).
 """
+import os
 import sys
 
 import numpy as np
@@ -26,7 +29,7 @@ def _say(*a):
 
 def test_plain_scft_trajectory_matches_oracle():
     m, D, N, f, alpha, ens, phidb, chi = (32, 32, 32), (4.0, 4.0, 4.0), 100, 0.5, 0.1, 1, 0.8, 20.0
-    K = 40
+    K = int(os.environ.get("SCFT_C0_ITERS", "40"))
     M = int(np.prod(m))
     rng = np.random.default_rng(20)
     x = (np.arange(m[0]) * D[0] / m[0])[:, None, None] * np.ones(m)
