@@ -453,6 +453,7 @@ def main():
     bstep = algorithmic_bytes_per_step(c["m"], p) * n_act
     achieved = bstep / (ms_step * 1e-3) / 1e9
     Wend = ctx.get_fields()
+    engine = ctx.engine_info.split(";")[0]
     ctx.close()
 
     traffic = None
@@ -467,7 +468,8 @@ def main():
         "dtype": "f32" if a.precision == "fp32" else "f64", "data": "synthetic (seeded lamellar/cylinder string)",
         "config": {"workload": "string method, 16 replicas x 64^3, N=128, D=6.0, chiN=18, f=0.45, phidb=0.9, "
                                "canonical, fix=(1,1)", "replicas": R, "grid": list(c["m"]), "N": c["N"],
-                   "props_per_iteration": props / a.steps, "parallelism": "1 GPU, all replicas batched"},
+                   "props_per_iteration": props / a.steps, "parallelism": "1 GPU, all replicas batched",
+                   "engine": engine},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "contour step (column, plane_mid, column, plane_end) over "
