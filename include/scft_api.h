@@ -70,10 +70,11 @@ typedef struct scft_params {
     double anderson_start;   /* simple mixing (lambda 0.05) until errs < this (<= 0: 1e-2;
                                 started much earlier, Anderson can land on another saddle
                                 point, e.g. the disordered one) */
-    int autotune;            /* 1: the first props times every kernel family (two-stage, LDS
-                                Stockham, line kernels) x replica-group streams (1-3) x graph
-                                replay on that batch and keeps the fastest. FP32 results then
-                                depend on the choice (the families round differently). */
+    int autotune;            /* DISABLED: scft_ctx_create returns SCFT_ERR_UNSUPPORTED for 1.
+                                Autotuned contexts crashed on the host in their first props
+                                with the cause not found. (Meant: the first props times every
+                                kernel family x replica-group streams x graph replay on that
+                                batch and keeps the fastest.) 0 = the measured defaults. */
     int ckpt;                /* 0: keep the propagators of every contour index (2(N+1) fields
                                 per replica, as density.h:293). K > 0: keep every K-th index
                                 (K ~ sqrt(N)) and recompute the rest in windows of K for the
@@ -106,7 +107,9 @@ scft_status scft_props_batch(scft_ctx *ctx, const double *W, double *phi, double
                              double *phis, int nrep);
 
 /* Same as scft_props_batch with device pointers for W (row stride 2M) and phi
- * (row stride 4M); lnQ and phis are host arrays. */
+ * (row stride 4M) on the context's device; lnQ and phis are host arrays. W is read on the
+ * context's own (non-blocking) stream: the caller completes the work that writes W before
+ * the call (e.g. synchronises its producer stream). phi is complete when the call returns. */
 scft_status scft_props_batch_device(scft_ctx *ctx, const double *dW, double *dphi,
                                     double *lnQ, double *phis, int nrep);
 

@@ -40,7 +40,9 @@ def build():
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liborc.so")
+        # ORC_LIB names another build of the oracle: the mutation tests point it at copies
+        # built with ORC_MUTANT set, each planting one known mistake.
+        path = os.environ.get("ORC_LIB") or os.path.join(_HERE, "liborc.so")
         if not os.path.exists(path):
             build()
         L = C.CDLL(path)

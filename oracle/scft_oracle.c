@@ -27,6 +27,14 @@
 
 typedef double complex cplx;
 
+/* Mutation switch for the oracle's own tests (tests/test_oracle_mutations.py): 0 builds
+ * the restatement; a value k > 0 plants one known mistake in the string iteration or
+ * FreeE (see the ORC_MUTANT sites) so the tests can show their closed forms catch it.
+ * liborc.so is always built with 0. */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 /* ------------------------------------------------------------------------------------ */
 /* 1-D mixed-radix Stockham FFT (unnormalised; forward uses exp(-2*pi*i*jk/n), like cuFFT) */
 /* ------------------------------------------------------------------------------------ */
@@ -559,7 +567,9 @@ double orc_edist(int M, const double *Wa, const double *Wb)
 static void arc_length(int strn, int M, double **W, double *x)
 {
     double Del[256], Delsum = 0;
-    for (int ii = 0; ii < strn - 1; ii++) { Del[ii] = orc_edist(M, W[ii], W[ii + 1]); Delsum += Del[ii]; }
+    /* the string metric is over W- only (scft.cu:351-355); mutant 3 adds W+ */
+    const int Mmetric = ORC_MUTANT == 3 ? 2 * M : M;
+    for (int ii = 0; ii < strn - 1; ii++) { Del[ii] = orc_edist(Mmetric, W[ii], W[ii + 1]); Delsum += Del[ii]; }
     for (int ii = 0; ii < strn - 1; ii++) Del[ii] /= Delsum;
     x[0] = 0; x[strn - 1] = 1;
     for (int ii = 1; ii < strn - 1; ii++) x[ii] = x[ii - 1] + Del[ii - 1];
@@ -569,7 +579,9 @@ void orc_string_iter(orc_grid *g, const orc_run *run, double **W, double **DEV, 
                      int k, double errTol, double *alf, orc_iter_report *rep)
 {
     const int M = g->M, strn = run->nrep;
-    const double lambda = 0.05, eps = 0.5, chi = run->chi;
+    const double lambda = ORC_MUTANT == 1 ? 0.5 : 0.05, eps = ORC_MUTANT == 6 ? 1.0 : 0.5, chi = run->chi;
+    const double lambda_redist = ORC_MUTANT == 8 ? 0.5 : lambda;
+    const double errs_norm = ORC_MUTANT == 2 ? M : 2.0 * M;
     double lnQ, phis[6], x[256], xref[256];
     int dospring = 1, doredist = 0, doperp = 0;
     if (run->dostring != 1) dospring = 0;
@@ -598,7 +610,7 @@ void orc_string_iter(orc_grid *g, const orc_run *run, double **W, double **DEV, 
     for (int ii = 0; ii < strn; ii++) {
         double S1 = 0.0;
         for (int r = 0; r < 2 * M; r++) S1 += DEV[ii][r] * DEV[ii][r];
-        rep->errs[ii] = pow(S1 / (2 * M), 0.5);
+        rep->errs[ii] = pow(S1 / errs_norm, 0.5);
     }
 
     for (int ii = 0; ii < strn; ii++) {
@@ -639,11 +651,12 @@ void orc_string_iter(orc_grid *g, const orc_run *run, double **W, double **DEV, 
         for (int ii = 0; ii < strn; ii++) rep->errs3[ii] = 0;
         for (int ii = 1; ii < strn - 1; ii++) {
             if (doiis[ii] == 1 && rep->normDER[ii] > 0 && rep->normDEV[ii] > 0) {
+                const double sgn = ORC_MUTANT == 7 ? -1.0 : 1.0;
                 for (int r = 0; r < M; r++)
-                    DEV[ii][r] -= (rep->dotWs[ii] / rep->normDER[ii]) * W[ii][6 * M + r] * rep->normDEV[ii];
+                    DEV[ii][r] -= sgn * (rep->dotWs[ii] / rep->normDER[ii]) * W[ii][6 * M + r] * rep->normDEV[ii];
                 double s = 0;
                 for (int r = 0; r < 2 * M; r++) s += DEV[ii][r] * DEV[ii][r];
-                rep->errs3[ii] = pow(s / (2 * M), 0.5);
+                rep->errs3[ii] = pow(s / errs_norm, 0.5);
             }
         }
         for (int ii = 0; ii < strn; ii++) {
@@ -677,7 +690,8 @@ void orc_string_iter(orc_grid *g, const orc_run *run, double **W, double **DEV, 
     for (int ii = 0; ii < strn; ii++) { rep->x[ii] = x[ii]; rep->xref[ii] = xref[ii]; }
 
     /* redistribution (scft.cu:745-757; density.h:511-528) */
-    if (doredist == 1) {
+    if (doredist == 1 && ORC_MUTANT != 4) {
+        const int i0 = ORC_MUTANT == 5 ? 0 : 1, i1 = ORC_MUTANT == 5 ? strn : strn - 1;
 #pragma omp parallel
         {
             double col[256], val[256];
@@ -685,8 +699,8 @@ void orc_string_iter(orc_grid *g, const orc_run *run, double **W, double **DEV, 
             for (int r = 0; r < M; r++) {
                 for (int ii = 0; ii < strn; ii++) col[ii] = W[ii][r];
                 orc_spline_value_at(strn, x, col, xref, val);
-                for (int ii = 1; ii < strn - 1; ii++)
-                    if (doiis[ii] == 1) W[ii][r] += (val[ii] - W[ii][r]) * lambda;
+                for (int ii = i0; ii < i1; ii++)
+                    if (doiis[ii] == 1) W[ii][r] += (val[ii] - W[ii][r]) * lambda_redist;
             }
         }
     }
@@ -713,7 +727,8 @@ double orc_free_energy(orc_grid *g, const orc_run *run, double **W, double **DEV
             const double *phiA = W[ii] + 2 * M, *phiB = W[ii] + 3 * M;
             for (int r = 0; r < M; r++) {
                 const double wa = W[ii][r] + W[ii][r + M], wb = W[ii][r + M] - W[ii][r];
-                FEs1[ii] += (chi * phiA[r] * phiB[r] - wa * phiA[r] - wb * phiB[r]) / M;
+                const double sa = ORC_MUTANT == 9 ? -1.0 : 1.0;
+                FEs1[ii] += (chi * phiA[r] * phiB[r] - sa * wa * phiA[r] - wb * phiB[r]) / M;
                 phctot[ii] += phiA[r] / (M * f);
             }
             FEs[ii] = FEs0[ii] + FEs1[ii];

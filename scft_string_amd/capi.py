@@ -112,6 +112,7 @@ class Context:
         self.M = int(np.prod(self.m))
         self.nrep = int(nrep)
         self.N = int(N)
+        self.device = int(device)
         prec = {"fp64": SCFT_FP64, "fp32": SCFT_FP32}[precision]
         p = Params(m=(C.c_int * 3)(*self.m), D=(C.c_double * 3)(*[float(v) for v in D]), N=int(N), f=float(f),
                    alpha=float(alpha), ens=int(ens), phidb=float(phidb), chi=float(chi), nrep=self.nrep,
@@ -170,7 +171,23 @@ class Context:
         return phi.reshape(n, 4, self.M), lnQ, phis
 
     def props_batch_device(self, W_dev, phi_dev, n):
-        """Device-pointer variant; W_dev, phi_dev are torch CUDA float64 tensors."""
+        """Device-pointer variant: W_dev (n rows of 2M) and phi_dev (n rows of 4M) are torch
+        float64 tensors on the context's GPU, C-contiguous. The library reads W_dev on its own
+        stream, so the producer's stream (torch's current stream) is synchronised first; phi_dev
+        is complete when this returns."""
+        import torch
+
+        n = int(n)
+        if not 0 < n <= self.nrep:
+            raise ValueError(f"props_batch_device: n = {n} outside 1..{self.nrep}")
+        for name, t, need in (("W_dev", W_dev, 2 * self.M * n), ("phi_dev", phi_dev, 4 * self.M * n)):
+            if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or t.device.index != self.device:
+                raise ValueError(f"props_batch_device: {name} must be a tensor on cuda:{self.device}")
+            if t.dtype != torch.float64 or not t.is_contiguous():
+                raise ValueError(f"props_batch_device: {name} must be contiguous float64")
+            if t.numel() < need:
+                raise ValueError(f"props_batch_device: {name} holds {t.numel()} values, needs {need}")
+        torch.cuda.current_stream(W_dev.device).synchronize()
         lnQ = np.empty(n)
         phis = np.empty((n, 5))
         self._chk(lib().scft_props_batch_device(self._h, C.c_void_p(W_dev.data_ptr()),

@@ -322,7 +322,7 @@ PropEngine<T>::~PropEngine()
         if (p) (void)hipFree(p);
     for (hipEvent_t e : ev_)
         if (e) (void)hipEventDestroy(e);
-    for (auto& g : graphs_) (void)hipGraphExecDestroy(g.exec);
+    clear_graphs();
     if (fork_) (void)hipEventDestroy(fork_);
     for (int i = 0; i + 1 < kMaxStreams; ++i) {
         if (join_[i]) (void)hipEventDestroy(join_[i]);
@@ -520,8 +520,7 @@ hipError_t PropEngine<T>::tune(const PropArgs<T>& a, int nb, const ChainConst& c
                 coltile_ = fams_[f].coltile;
                 nstreams_ = ns;
                 use_graph_ = gph == 1;
-                for (auto& g : graphs_) (void)hipGraphExecDestroy(g.exec);
-                graphs_.clear();
+                clear_graphs();
                 float t = -1;
                 for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
                     if ((e = hipEventRecord(e0, st)) != hipSuccess) break;
@@ -556,8 +555,7 @@ hipError_t PropEngine<T>::tune(const PropArgs<T>& a, int nb, const ChainConst& c
     path_ = fams_[best.fam].name;
     nstreams_ = best.streams;
     use_graph_ = best.graph;
-    for (auto& g : graphs_) (void)hipGraphExecDestroy(g.exec);
-    graphs_.clear();
+    clear_graphs();
     tuned_ = true;
     return hipSuccess;
 }
@@ -819,25 +817,48 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
 template <typename T>
 hipError_t PropEngine<T>::contour(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st)
 {
-    if (!use_graph_) return contour_launch(a, nb, cc, st);
+    if (!use_graph_) return contour_launch(a, nb, cc, st, fork_, join_);
     for (auto& g : graphs_)
         if (g.nb == nb && g.st == st) return hipGraphLaunch(g.exec, st);
+    GraphEntry ge{nb, st, nullptr, {}};
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < kMaxStreams && e == hipSuccess; ++i)
+        e = hipEventCreateWithFlags(&ge.ev[i], hipEventDisableTiming);
     hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return e;
-    hipError_t el = contour_launch(a, nb, cc, st);
-    e = hipStreamEndCapture(st, &graph);
-    if (el != hipSuccess) e = el;
-    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipError_t el = contour_launch(a, nb, cc, st, ge.ev[0], ge.ev + 1);
+        e = hipStreamEndCapture(st, &graph);
+        if (el != hipSuccess) e = el;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
     if (graph) (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) return e;
-    graphs_.push_back({nb, st, exec});
-    return hipGraphLaunch(exec, st);
+    if (e == hipSuccess) e = hipGraphLaunch(ge.exec, st);
+    if (e != hipSuccess && !ge.exec) {
+        for (hipEvent_t ev : ge.ev)
+            if (ev) (void)hipEventDestroy(ev);
+        return e;
+    }
+    graphs_.push_back(ge);
+    return e;
+}
+
+// Destroys every captured contour loop with its events (after the work that used them is
+// complete: the callers synchronise first).
+template <typename T>
+void PropEngine<T>::clear_graphs()
+{
+    for (auto& g : graphs_) {
+        (void)hipGraphExecDestroy(g.exec);
+        for (hipEvent_t ev : g.ev)
+            if (ev) (void)hipEventDestroy(ev);
+    }
+    graphs_.clear();
 }
 
 template <typename T>
-hipError_t PropEngine<T>::contour_launch(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st)
+hipError_t PropEngine<T>::contour_launch(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st, hipEvent_t fork,
+                                         const hipEvent_t* join)
 {
     const int ng = std::min(nstreams_, nb);
     if (ng > 1) {
@@ -859,15 +880,15 @@ hipError_t PropEngine<T>::contour_launch(PropArgs<T> a, int nb, const ChainConst
             gs[i] = i == 0 ? st : sx_[i - 1];
             first += cnt;
         }
-        hipError_t e = hipEventRecord(fork_, st);
-        for (int i = 1; i < ng && e == hipSuccess; ++i) e = hipStreamWaitEvent(gs[i], fork_, 0);
+        hipError_t e = hipEventRecord(fork, st);
+        for (int i = 1; i < ng && e == hipSuccess; ++i) e = hipStreamWaitEvent(gs[i], fork, 0);
         if (e != hipSuccess) return e;
         for (int i = 0; i < ng; ++i) begin_(g[i], gs[i]);
         for (int s = 0; s < cc.N; ++s)
             for (int i = 0; i < ng; ++i) step_(g[i], s, gs[i]);
         for (int i = 1; i < ng && e == hipSuccess; ++i) {
-            e = hipEventRecord(join_[i - 1], gs[i]);
-            if (e == hipSuccess) e = hipStreamWaitEvent(st, join_[i - 1], 0);
+            e = hipEventRecord(join[i - 1], gs[i]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(st, join[i - 1], 0);
         }
         if (e != hipSuccess) return e;
     } else {

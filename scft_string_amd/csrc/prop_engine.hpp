@@ -89,6 +89,8 @@ public:
     double loop_ms() const;
 
 private:
+    static constexpr int kMaxStreams = 4;
+    static constexpr int kDefaultStreams = 2;
     PropArgs<T> args(int nb, const ChainConst& cc) const;
     hipError_t tune(const PropArgs<T>& a, int nb, const ChainConst& cc, hipStream_t st);
     struct Family {
@@ -104,7 +106,10 @@ private:
     std::string tune_log_;
     hipError_t ensure_streams(int n);
     hipError_t contour(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st);
-    hipError_t contour_launch(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st);
+    // fork / join: the events that order the replica-group streams after st and st after
+    // them (the engine's own for direct launches, the graph entry's own under capture)
+    hipError_t contour_launch(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st, hipEvent_t fork,
+                              const hipEvent_t* join);
     // checkpointed store: recompute contour indices lo..hi into window w (entries
     // ck_window(w) + s - lo), from the checkpoint at or below lo
     void window(PropArgs<T> a, int w, int lo, int hi, hipStream_t st);
@@ -112,12 +117,18 @@ private:
     int store_index(int s, const ChainConst& cc) const { return ck_ ? ck_index(s, cc.N, cc.Nh, ck_) : s; }
     hipError_t windowed_density(const PropArgs<T>& a, int nb, const ChainConst& cc, double* dphi,
                                 long long phistride, const int* d_slot2rep, hipStream_t st);
+    // A captured contour loop. Its fork / join events are created for that capture and
+    // used by nothing else: an event recorded inside a capture refers to that capture's
+    // graph nodes, so one recorded in an earlier capture (whose graph is gone) must never
+    // order a later capture or a direct launch.
     struct GraphEntry {
         int nb;
         hipStream_t st;
         hipGraphExec_t exec;
+        hipEvent_t ev[kMaxStreams];  // [0] fork, [1..] join
     };
     std::vector<GraphEntry> graphs_;
+    void clear_graphs();
     bool use_graph_ = true;
     bool pencil_ct_ = true;  // line kernels compiled for the axis length where one exists
     int m_[3] = {0, 0, 0};
@@ -152,8 +163,6 @@ private:
     size_t bytes_ = 0;
     bool timing_ = false;
     hipEvent_t ev_[2] = {nullptr, nullptr};
-    static constexpr int kMaxStreams = 4;
-    static constexpr int kDefaultStreams = 2;
     int nstreams_ = 1;
     hipStream_t sx_[kMaxStreams - 1] = {};
     hipEvent_t fork_ = nullptr;

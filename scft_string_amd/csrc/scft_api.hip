@@ -399,6 +399,13 @@ scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
     if (p->ens == 1 && p->phidb + p->phihA > 1.0) return SCFT_ERR_INVALID;
     if (p->anderson < 0 || p->anderson > 40 || (p->anderson > 0 && p->dostring != 0)) return SCFT_ERR_INVALID;
     if (p->ckpt < 0) return SCFT_ERR_INVALID;
+    // The autotuner (PropEngine::tune) is switched off: autotuned contexts crashed on the host
+    // (a segmentation fault in the first props) with the cause not found. Refused before any
+    // device work, so the rest of the library is unaffected.
+    if (p->autotune) {
+        fprintf(stderr, "scft_ctx_create: autotune = 1 is disabled in this build (host crash, cause not found)\n");
+        return SCFT_ERR_UNSUPPORTED;
+    }
 
     if (hipSetDevice(p->device) != hipSuccess) return SCFT_ERR_HIP;
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return SCFT_ERR_HIP;

@@ -42,3 +42,20 @@ def test_python_layer_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(capi, "_LIB", None)
     with pytest.raises(ImportError):
         capi.lib()
+
+
+def test_autotune_is_refused_before_any_device_work():
+    """autotune = 1 is switched off: scft_ctx_create refuses it while validating the
+    parameters, before it selects a device (so this runs without a GPU)."""
+    import ctypes as C
+
+    from scft_string_amd import capi
+
+    if not os.path.exists(capi.LIB_PATH):
+        pytest.skip("libscft_hip.so not built")
+    p = capi.Params(m=(C.c_int * 3)(16, 16, 16), D=(C.c_double * 3)(4.0, 4.0, 4.0), N=20, f=0.5, alpha=0.1, ens=1,
+                    phidb=0.8, chi=15.0, nrep=1, precision=capi.SCFT_FP64, fix=(C.c_int * 2)(0, 0), dostring=0,
+                    autotune=1)
+    h = C.c_void_p()
+    assert capi.lib().scft_ctx_create(C.byref(p), C.byref(h)) == capi.SCFT_ERR_UNSUPPORTED
+    assert not h.value

@@ -11,7 +11,10 @@ import pytest
 
 from scft_string_amd import Context
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skip(reason="autotune is disabled: autotuned contexts crashed on the host (segmentation "
+                                      "fault in the first props) and the cause is not found; scft_ctx_create "
+                                      "refuses autotune = 1")]
 
 
 def _say(*a):

@@ -19,6 +19,7 @@ import numpy as np
 import pytest
 
 from oracle import orc
+from tests.uniform_string import Blend, replay_main
 
 pytestmark = pytest.mark.gpu
 
@@ -92,6 +93,42 @@ def test_string_run_matches_oracle(tmp_path, ckpt):
     _say(f"driver FEs {got[:, 1]} oracle {ref[:, 1]}")
     assert np.max(np.abs(got[:, 1:] - ref[:, 1:])) <= 2e-10
     assert np.max(np.abs(got[:, 0] - ref[:, 0])) <= 2e-6   # printed with 6 decimals
+
+
+def test_uniform_string_files_match_closed_form(tmp_path):
+    """main()'s cadence pinned by closed forms: a spatially uniform grand-canonical string
+    (every quantity a scalar per replica, tests/uniform_string.py) run for 52 iterations.
+    FreeE runs at k = 0, 50 and 51 and relaxes W+ of every replica, the fixed ends included,
+    so the cadence shows in the check file (k = 51, errs / errs3 with 12 decimals, the FE of
+    the FreeE at k = 50) and in the FEs file of the final FreeE."""
+    m, D, R, maxiter, prerelax = (8, 8, 8), (4.0, 4.0, 4.0), 5, 52, 200
+    N, f, alpha, phidb, chi, ens = 20, 0.5, 0.1, 0.5, 12.0, 2
+    M = int(np.prod(m))
+    a0 = np.array([-1.2, -0.7, -0.1, 0.3, 1.4])
+    for i in range(R):
+        _write_win(os.path.join(tmp_path, f"win{i}"), np.stack([np.full(M, a0[i]), np.zeros(M)]))
+    _write_input(tmp_path, chi=chi, f=f, ens=ens, phidb=phidb, N=N, m=m, D=D, fix=(1, 1), readwin=1,
+                 justFE=0, dostring=1, strn=R, maxiter=maxiter, prerelax=prerelax, precision=64)
+    _say("driver: uniform grand-canonical string, 52 iterations")
+    _run(tmp_path)
+    want = replay_main(Blend(N, f, alpha, phidb, chi, ens), a0, np.zeros(R), maxiter, prerelax, fix=(1, 1), M=M)
+    with open(os.path.join(tmp_path, "check")) as fh:
+        lines = [l.split() for l in fh.read().strip().splitlines()]
+    k, errtot, FEprev, errs, errs3 = want["check"]
+    assert int(lines[0][0]) == k == maxiter - 1
+    assert abs(float(lines[0][1]) - errtot) <= 1e-6 and float(lines[0][2]) == 0.05
+    assert abs(float(lines[0][3]) - FEprev) <= 1e-6
+    got = np.array([[float(v) for v in l[1:]] for l in lines[1:]])
+    assert got.shape == (R, 2)
+    assert np.max(np.abs(got[:, 0] - errs)) <= 1e-11 and np.max(np.abs(got[:, 1] - errs3)) <= 1e-11
+    fes = np.loadtxt(os.path.join(tmp_path, "FEs"))
+    FE, FEs, ph, alf = want["final"]
+    _say(f"uniform string: FEs {fes[:, 1]} closed form {FEs}")
+    assert np.max(np.abs(fes[:, 1] - FEs)) <= 2e-10 and np.max(np.abs(fes[:, 2] - ph)) <= 2e-10
+    assert np.max(np.abs(fes[:, 0] - alf / alf[-1])) <= 1e-6
+    for i in range(1, R - 1):   # interior rows written at the end; uniform fields
+        w = np.loadtxt(os.path.join(tmp_path, f"win{i}"))
+        assert np.max(np.abs(w[:, 0] - want["a"][i])) <= 1e-6 and np.max(np.abs(w[:, 1] - want["b"][i])) <= 1e-6
 
 
 def test_relaxbox_key(tmp_path):
