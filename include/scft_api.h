@@ -199,6 +199,39 @@ scft_status scft_relax_box(scft_ctx *ctx, int axes, double stressTol, double err
                            int maxFieldIter, int maxBoxes, double D[3], double *dFdD,
                            int *boxes);
 
+/* The string's spline on host rows, chunk by chunk, as class density exposes it
+ * (density.h:462-528, called from scft.cu:643-650, 746-751, 1011). ww[ii] are the nrep
+ * replica rows (host; W- of replica ii from index 0), a chunk is grid points r0 .. r0+n-1.
+ *
+ * Replaces setupSplineBuffers (density.h:462-476): device and pinned staging for chunks of
+ * up to chunkM points of every replica (min(chunkM, M) is kept). Needed once before the
+ * chunk calls; calling it again resizes. */
+scft_status scft_spline_setup(scft_ctx *ctx, int chunkM);
+/* Replace setSplineX / setSplineXRef (density.h:478-488): nodes x[nrep] (strictly
+ * increasing; the reference divides by zero otherwise) and evaluation points xref[nrep] of
+ * the following chunk calls (copied). */
+scft_status scft_spline_set_x(scft_ctx *ctx, const double *x);
+scft_status scft_spline_set_xref(scft_ctx *ctx, const double *xref);
+/* Replaces splineDerivsChunked (density.h:491-509): out[ii][r] = derivative at x[ii] of the
+ * natural cubic spline through (x[j], ww[j][r]), for every ii with doiis[ii] == 1 and r in
+ * the chunk. Other rows of out are not touched. */
+scft_status scft_spline_derivs(scft_ctx *ctx, const double *const *ww, int r0, int n, const int *doiis,
+                               double *const *out);
+/* Replaces splineRedistChunked (density.h:511-528): ww[ii][r] += lambda (s(xref[ii]) - ww[ii][r])
+ * with s the spline through (x[j], ww[j][r]), for interior ii (0 < ii < nrep - 1) with
+ * doiis[ii] == 1 and r in the chunk; the spline is fitted to the values before the update. */
+scft_status scft_spline_redist(scft_ctx *ctx, double *const *ww, int r0, int n, const int *doiis, double lambda);
+
+/* Weights of the natural cubic spline through (x[i], a[i]), i < nrep (cubefit_chunk_g,
+ * GPU_kernels.h:232-284), at the points xe[erows[j]], j < ne: G[j][i] (row-major, ne x nrep)
+ * is the derivative (deriv = 1) or the value (deriv = 0) there per unit a[i]; a point lies
+ * on the interval of the last node strictly below it (GPU_kernels.h:291-297). x strictly
+ * increasing, 2 <= nrep <= 64. Host only, no device work: the string kernels apply these
+ * weights to every grid column. No counterpart in the reference, which refits the spline
+ * at every grid point. */
+scft_status scft_spline_weights(int nrep, const double *x, const double *xe, const int *erows, int ne,
+                                int deriv, double *G);
+
 /* Diagnostics for tests: after a props call on slot `slot` of the scratch batch,
  * qq[s] = <q1'[s] q2'[N-s]> for s = 0..N (the A4 symmetry invariant, with the
  * mean-field factoring left in; all entries equal QC' in exact arithmetic). */

@@ -16,6 +16,7 @@ INC = os.path.join(ROOT, "include")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libscft_hip.so")
 DRIVER = os.path.join(PKG, "scft_drive")
+DROPIN = os.path.join(PKG, "scft_dropin_example")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SCFT_OFFLOAD_ARCH", "gfx950")
@@ -71,12 +72,15 @@ def build_plain(verbose=True):
 
 
 def build_driver(verbose=True):
-    src = os.path.join(CSRC, "driver", "scft_drive.cpp")
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", INC, "-I", os.path.join(CSRC, "driver"), src, "-o", DRIVER,
-           f"-L{PKG}", "-lscft_hip", f"-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
-    subprocess.run(cmd, check=True)
-    if verbose:
-        print("built", DRIVER)
+    """The C++ host driver (main() of scft.cu) and the drop-in example (scft.cu-shaped caller
+    of the density mirror), both plain g++ against the C-ABI."""
+    for name, out in (("scft_drive.cpp", DRIVER), ("dropin_example.cpp", DROPIN)):
+        src = os.path.join(CSRC, "driver", name)
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", INC, "-I", os.path.join(CSRC, "driver"), src, "-o", out,
+               f"-L{PKG}", "-lscft_hip", f"-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+        subprocess.run(cmd, check=True)
+        if verbose:
+            print("built", out)
     return DRIVER
 
 

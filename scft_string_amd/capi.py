@@ -21,7 +21,8 @@ EXPORTED = [
     "scft_get_tangent", "scft_solve_w_plus", "scft_string_iterate", "scft_free_energy",
     "scft_debug_qq_invariant", "scft_debug_time_props", "scft_fft_path", "scft_dlnq_dbox",
     "scft_box_stress", "scft_set_box", "scft_relax_box", "scft_iterate_concurrent", "scft_engine_info",
-    "scft_backward",
+    "scft_backward", "scft_spline_setup", "scft_spline_set_x", "scft_spline_set_xref", "scft_spline_derivs",
+    "scft_spline_redist", "scft_spline_weights",
 ]
 _STRING_RESULT = ("scft_last_error", "scft_fft_path", "scft_engine_info")
 
@@ -89,6 +90,12 @@ def lib():
         L.scft_relax_box.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp, ip]
         L.scft_iterate_concurrent.argtypes = [C.POINTER(vp), C.c_int, ip, C.c_double, C.POINTER(dp),
                                               C.POINTER(IterReport)]
+        L.scft_spline_setup.argtypes = [vp, C.c_int]
+        L.scft_spline_set_x.argtypes = [vp, dp]
+        L.scft_spline_set_xref.argtypes = [vp, dp]
+        L.scft_spline_derivs.argtypes = [vp, C.POINTER(dp), C.c_int, C.c_int, ip, C.POINTER(dp)]
+        L.scft_spline_redist.argtypes = [vp, C.POINTER(dp), C.c_int, C.c_int, ip, C.c_double]
+        L.scft_spline_weights.argtypes = [C.c_int, dp, dp, ip, C.c_int, C.c_int, dp]
         for name in EXPORTED:
             if name not in _STRING_RESULT:
                 getattr(L, name).restype = C.c_int
@@ -99,6 +106,26 @@ def lib():
 def _dp(a):
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _rows(arr):
+    """double** over the rows of a C-contiguous float64 2-D array"""
+    dp = C.POINTER(C.c_double)
+    return (dp * arr.shape[0])(*[arr[i].ctypes.data_as(dp) for i in range(arr.shape[0])])
+
+
+def spline_weights(x, xe, erows, deriv):
+    """scft_spline_weights: (len(erows), len(x)) weights of the natural spline through x at
+    xe[erows] (derivative or value). Host only."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    xe = np.ascontiguousarray(xe, dtype=np.float64)
+    er = np.ascontiguousarray(erows, dtype=np.int32)
+    G = np.zeros((len(er), len(x)))
+    rc = lib().scft_spline_weights(len(x), _dp(x), _dp(xe), er.ctypes.data_as(C.POINTER(C.c_int)), len(er),
+                                   int(bool(deriv)), _dp(G))
+    if rc != SCFT_OK:
+        raise ScftError(rc, "scft_spline_weights: bad arguments")
+    return G
 
 
 class Context:
@@ -193,6 +220,30 @@ class Context:
         self._chk(lib().scft_props_batch_device(self._h, C.c_void_p(W_dev.data_ptr()),
                                                 C.c_void_p(phi_dev.data_ptr()), _dp(lnQ), _dp(phis), n))
         return lnQ, phis
+
+    # ------------------------------------------------ chunked host-row spline (density.h:462-528)
+    def spline_setup(self, chunk):
+        self._chk(lib().scft_spline_setup(self._h, int(chunk)))
+
+    def spline_set_x(self, x):
+        self._chk(lib().scft_spline_set_x(self._h, _dp(np.ascontiguousarray(x, dtype=np.float64))))
+
+    def spline_set_xref(self, xref):
+        self._chk(lib().scft_spline_set_xref(self._h, _dp(np.ascontiguousarray(xref, dtype=np.float64))))
+
+    def spline_derivs(self, ww, r0, n, doiis, out):
+        """ww, out: (nrep, >= r0 + n) float64 host rows; out rows with doiis[ii] == 1 get the
+        spline derivative at the nodes over points r0 .. r0+n-1."""
+        assert ww.dtype == np.float64 and ww.flags["C_CONTIGUOUS"] and out.flags["C_CONTIGUOUS"]
+        d = np.ascontiguousarray(doiis, dtype=np.int32)
+        self._chk(lib().scft_spline_derivs(self._h, _rows(ww), int(r0), int(n), d.ctypes.data_as(C.POINTER(C.c_int)),
+                                           _rows(out)))
+
+    def spline_redist(self, ww, r0, n, doiis, lam):
+        assert ww.dtype == np.float64 and ww.flags["C_CONTIGUOUS"]
+        d = np.ascontiguousarray(doiis, dtype=np.int32)
+        self._chk(lib().scft_spline_redist(self._h, _rows(ww), int(r0), int(n), d.ctypes.data_as(C.POINTER(C.c_int)),
+                                           float(lam)))
 
     def qq_invariant(self, slot=0):
         out = np.empty(self.N + 1)

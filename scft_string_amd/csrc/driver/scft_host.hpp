@@ -20,7 +20,10 @@ inline void check(scft_ctx* ctx, scft_status s, const char* what)
     if (s != SCFT_OK) throw error(s, std::string(what) + ": " + scft_last_error(ctx));
 }
 
-// Mirrors class density (density.h:225-548): one device context for every replica.
+// Mirrors class density (density.h:225-548): one device context for every replica. The
+// reference's FreeE, solve_field0 and solve_field (scft.cu:516-853) call only props and the
+// spline methods below, so they compile against this class unchanged
+// (dropin_example.cpp in this folder is such a caller).
 class density {
 public:
     explicit density(const scft_params& p) : p_(p)
@@ -44,6 +47,22 @@ public:
         check(ctx_, scft_props_batch(ctx_, w, phi.data(), lnQ, ph, 1), "props");
         std::copy(phi.begin(), phi.end(), w + 2 * M_);
         for (int i = 0; i < 5; ++i) phis[i] = ph[i];
+    }
+
+    // The string's spline on host rows (density.h:462-528), same signatures. As in the
+    // reference the kernels take the nodes and evaluation points from setSplineX /
+    // setSplineXRef; the x and xref arguments of the chunk calls are not read.
+    void setupSplineBuffers(int chunkM) { check(ctx_, scft_spline_setup(ctx_, chunkM), "setupSplineBuffers"); }
+    void setSplineX(double* x) { check(ctx_, scft_spline_set_x(ctx_, x), "setSplineX"); }
+    void setSplineXRef(double* xref) { check(ctx_, scft_spline_set_xref(ctx_, xref), "setSplineXRef"); }
+    void splineDerivsChunked(double** ww, double* /*x*/, int r0, int chunkSize, int* doiis, double** out)
+    {
+        check(ctx_, scft_spline_derivs(ctx_, ww, r0, chunkSize, doiis, out), "splineDerivsChunked");
+    }
+    void splineRedistChunked(double** ww, double* /*x*/, double* /*xref*/, int r0, int chunkSize, int* doiis,
+                             const double lambda)
+    {
+        check(ctx_, scft_spline_redist(ctx_, ww, r0, chunkSize, doiis, lambda), "splineRedistChunked");
     }
 
     scft_ctx* ctx() { return ctx_; }

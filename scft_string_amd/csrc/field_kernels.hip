@@ -139,80 +139,57 @@ void launch_finalize(const double* part, int nslots, int nblk, int ncomp, double
 }
 
 // ---------------------------------------------------------------- natural cubic spline
-// The data-independent parts of cubefit_chunk_g (GPU_kernels.h:253-265): h, l, u, plus
-// the evaluation intervals of eval_spline_*_chunk_g (GPU_kernels.h:291-297), shared by
-// every grid point.
-struct SplineShared {
-    double x[SPLINE_RMAX], h[SPLINE_RMAX], l[SPLINE_RMAX], u[SPLINE_RMAX];
-    int nm[SPLINE_RMAX];
-    double dx[SPLINE_RMAX];
-};
+// At fixed nodes x the natural cubic spline of one grid point's column a[0..R) (cubefit_chunk_g,
+// GPU_kernels.h:232-284) and its derivative or value at any point (eval_spline_*_chunk_g,
+// GPU_kernels.h:286-329) are linear in a. The host turns x (and the evaluation points) into
+// one R x R weight matrix per launch (spline_weights, scft_api.hip); here every column r of
+// the replica rows is multiplied by it: out[erows[j]][r] = sum_i G[j][i] in[i][r].
+// A workgroup stages RM_TR consecutive columns of all R rows in LDS (coalesced row loads),
+// then each wave produces whole output rows from LDS, the weights broadcast across the wave.
+// The whole tile is read before any of it is written, so out may be in (in-place update).
+constexpr int RM_TR = 64;  // columns per tile: one wave's lanes
 
-__device__ void spline_setup(SplineShared& S, int R, const double* __restrict__ x, const double* __restrict__ xe)
+__global__ void __launch_bounds__(FK_NT) k_row_mix(int R, long long M, const double* __restrict__ G,
+                                                   const int* __restrict__ erows, int ne, const double* in,
+                                                   long long istride, double* out, long long ostride)
 {
-    const int n = R - 1;
-    for (int i = threadIdx.x; i < R; i += FK_NT) S.x[i] = x[i];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < R; ++i) S.h[i] = S.l[i] = S.u[i] = 0.0;
-        for (int i = 0; i <= n - 1; ++i) S.h[i] = S.x[i + 1] - S.x[i];
-        S.l[0] = 1.0;
-        S.u[0] = 0.0;
-        for (int i = 1; i <= n - 1; ++i) {
-            S.l[i] = 2.0 * (S.x[i + 1] - S.x[i - 1]) - S.h[i - 1] * S.u[i - 1];
-            S.u[i] = S.h[i] / S.l[i];
-        }
-        S.l[n] = 1.0;
-        for (int e = 0; e < R; ++e) {
-            const double xx = xe[e];
-            int nm = 0;
-            for (int i = 0; i < R; ++i) {
-                if (S.x[i] < xx) nm = i;
-                else break;
+    extern __shared__ double lds[];
+    double* S = lds;                 // [R][RM_TR]
+    double* Gs = lds + R * RM_TR;    // [ne][R]
+    for (int t = threadIdx.x; t < ne * R; t += FK_NT) Gs[t] = G[t];
+    const int lane = threadIdx.x % RM_TR, grp = threadIdx.x / RM_TR;
+    constexpr int NG = FK_NT / RM_TR;
+    for (long long r0 = (long long)blockIdx.x * RM_TR; r0 < M; r0 += (long long)gridDim.x * RM_TR) {
+        const long long r = r0 + lane;
+        __syncthreads();  // the previous tile's LDS reads are done (and Gs is written)
+        for (int i = grp; i < R; i += NG) S[i * RM_TR + lane] = r < M ? in[(size_t)i * istride + r] : 0.0;
+        __syncthreads();
+        if (r < M)
+            for (int j = grp; j < ne; j += NG) {
+                const double* g = Gs + j * R;
+                double v = 0.0;
+                for (int i = 0; i < R; ++i) v += g[i] * S[i * RM_TR + lane];
+                out[(size_t)erows[j] * ostride + r] = v;
             }
-            if (nm > n - 1) nm = n - 1;  // only reachable for x that is not increasing
-            if (nm < 0) nm = 0;
-            S.nm[e] = nm;
-            S.dx[e] = xx - S.x[nm];
-        }
     }
-    __syncthreads();
 }
 
-// c coefficients of one grid point's natural spline through a[0..R) (GPU_kernels.h:254-275)
-__device__ __forceinline__ void spline_c(const SplineShared& S, int R, const double* a, double* c)
+size_t row_mix_lds_bytes(int R, int ne) { return sizeof(double) * ((size_t)R * RM_TR + (size_t)ne * R); }
+
+hipError_t row_mix_setup()
 {
-    const int n = R - 1;
-    double z[SPLINE_RMAX];
-    z[0] = 0.0;
-    for (int i = 1; i <= n - 1; ++i) {
-        const double A = 3.0 * (a[i + 1] - a[i]) / S.h[i] - 3.0 * (a[i] - a[i - 1]) / S.h[i - 1];
-        z[i] = (A - S.h[i - 1] * z[i - 1]) / S.l[i];
-    }
-    c[n] = 0.0;
-    for (int j = n - 1; j >= 0; --j) c[j] = z[j] - S.u[j] * c[j + 1];
+    return hipFuncSetAttribute((const void*)k_row_mix, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)row_mix_lds_bytes(SPLINE_RMAX, SPLINE_RMAX));
 }
 
-__global__ void __launch_bounds__(FK_NT) k_spline_deriv(int R, long long M, const double* __restrict__ x,
-                                                        const int* __restrict__ active, const double* __restrict__ W,
-                                                        double* __restrict__ dWda)
+void launch_row_mix(int R, long long M, const double* d_G, const int* d_erows, int ne, const double* in,
+                    long long istride, double* out, long long ostride, hipStream_t st)
 {
-    __shared__ SplineShared S;
-    spline_setup(S, R, x, x);
-    GRID_STRIDE(r, M)
-    {
-        double a[SPLINE_RMAX], c[SPLINE_RMAX];
-        for (int i = 0; i < R; ++i) a[i] = W[(size_t)i * 2 * M + r];
-        spline_c(S, R, a, c);
-        for (int e = 0; e < R; ++e) {
-            if (!active[e]) continue;
-            const int nm = S.nm[e];
-            const double h = S.h[nm], dx = S.dx[e];
-            const double b = (a[nm + 1] - a[nm]) / h - h * (c[nm + 1] + 2.0 * c[nm]) / 3.0;
-            const double d = (c[nm + 1] - c[nm]) / (3.0 * h);
-            dWda[(size_t)e * M + r] = b + 2.0 * dx * c[nm] + 3.0 * dx * dx * d;
-        }
-    }
+    if (ne <= 0 || M <= 0) return;
+    const long long tiles = (M + RM_TR - 1) / RM_TR;
+    const int grid = (int)(tiles < 8192 ? tiles : 8192);
+    hipLaunchKernelGGL(k_row_mix, dim3(grid), dim3(FK_NT), row_mix_lds_bytes(R, ne), st, R, M, d_G, d_erows, ne, in,
+                       istride, out, ostride);
 }
 
 // part [slot][nblk][2] = (sum dWda^2, sum DEV- dWda)  (scft.cu:649-667)
@@ -237,14 +214,6 @@ __global__ void __launch_bounds__(FK_NT) k_tangent_dots(const int* __restrict__ 
         part[((size_t)slot * gridDim.x + blockIdx.x) * 2 + 0] = s1;
         part[((size_t)slot * gridDim.x + blockIdx.x) * 2 + 1] = s2;
     }
-}
-
-void launch_spline_deriv(int R, long long M, const double* d_x, const int* d_active, const double* W,
-                         double* dWda, hipStream_t st)
-{
-    const int nblk = (int)((M + FK_NT - 1) / FK_NT);
-    hipLaunchKernelGGL(k_spline_deriv, dim3(nblk < 2048 ? nblk : 2048), dim3(FK_NT), 0, st, R, M, d_x, d_active, W,
-                       dWda);
 }
 
 void launch_tangent_dots(const int* d_slot2rep, int nb, long long M, const double* dWda, const double* DEV,
@@ -291,37 +260,6 @@ void launch_project_mix(int R, long long M, const double* d_cproj, double lambda
                         double* W, double* part, int nblk, hipStream_t st)
 {
     hipLaunchKernelGGL(k_project_mix, dim3(nblk, R), dim3(FK_NT), 0, st, M, d_cproj, lambda, dWda, DEV, W, part);
-}
-
-__global__ void __launch_bounds__(FK_NT) k_redistribute(int R, long long M, const double* __restrict__ x,
-                                                        const double* __restrict__ xref, const int* __restrict__ active,
-                                                        double lambda, double* __restrict__ W)
-{
-    __shared__ SplineShared S;
-    spline_setup(S, R, x, xref);
-    GRID_STRIDE(r, M)
-    {
-        double a[SPLINE_RMAX], c[SPLINE_RMAX];
-        for (int i = 0; i < R; ++i) a[i] = W[(size_t)i * 2 * M + r];
-        spline_c(S, R, a, c);
-        for (int e = 1; e < R - 1; ++e) {
-            if (!active[e]) continue;
-            const int nm = S.nm[e];
-            const double h = S.h[nm], dx = S.dx[e];
-            const double b = (a[nm + 1] - a[nm]) / h - h * (c[nm + 1] + 2.0 * c[nm]) / 3.0;
-            const double d = (c[nm + 1] - c[nm]) / (3.0 * h);
-            const double v = a[nm] + dx * b + dx * dx * c[nm] + dx * dx * dx * d;
-            W[(size_t)e * 2 * M + r] = a[e] + (v - a[e]) * lambda;
-        }
-    }
-}
-
-void launch_redistribute(int R, long long M, const double* d_x, const double* d_xref, const int* d_active,
-                         double lambda, double* W, hipStream_t st)
-{
-    const int nblk = (int)((M + FK_NT - 1) / FK_NT);
-    hipLaunchKernelGGL(k_redistribute, dim3(nblk < 2048 ? nblk : 2048), dim3(FK_NT), 0, st, R, M, d_x, d_xref,
-                       d_active, lambda, W);
 }
 
 // ---------------------------------------------------------------- free energy sums

@@ -25,10 +25,13 @@ void launch_edist(int R, long long M, const double* W, double* part, int nblk, h
 // finalize: out[slot*ncomp + c] = sum over nblk of part[(slot*nblk + b)*ncomp + c]
 void launch_finalize(const double* part, int nslots, int nblk, int ncomp, double* out, hipStream_t st);
 
-// spline derivative dW-/dalpha at the nodes (GPU_kernels.h:232-306) for every r; writes
-// dWda[ii] for active[ii]
-void launch_spline_deriv(int R, long long M, const double* d_x, const int* d_active, const double* W,
-                         double* dWda, hipStream_t st);
+// out[erows[j]][r] = sum_i G[j][i] in[i][r] for j < ne, r < M (G: [ne][R] device, row
+// strides istride / ostride; out may equal in). The spline tangent and redistribution of
+// the string (GPU_kernels.h:232-329) with the weights of spline_weights. R <= SPLINE_RMAX.
+void launch_row_mix(int R, long long M, const double* d_G, const int* d_erows, int ne, const double* in,
+                    long long istride, double* out, long long ostride, hipStream_t st);
+// its dynamic LDS bound (call once per process before the first launch)
+hipError_t row_mix_setup();
 
 // part [slot][nblk][2] = (sum dWda^2, sum DEV- dWda) per slot (scft.cu:649-667)
 void launch_tangent_dots(const int* d_slot2rep, int nb, long long M, const double* dWda, const double* DEV,
@@ -39,11 +42,6 @@ void launch_tangent_dots(const int* d_slot2rep, int nb, long long M, const doubl
 // sum DEV-' dWda)
 void launch_project_mix(int R, long long M, const double* d_cproj, double lambda, const double* dWda,
                         double* DEV, double* W, double* part, int nblk, hipStream_t st);
-
-// redistribution (scft.cu:745-757, density.h:511-528): for interior active replicas,
-// W- += (spline_x(W-)(xref) - W-) lambda
-void launch_redistribute(int R, long long M, const double* d_x, const double* d_xref, const int* d_active,
-                         double lambda, double* W, hipStream_t st);
 
 // FreeE sums (scft.cu:823-828) for nb slots: part [slot][nblk][2] =
 // (sum chi phiA phiB - wa phiA - wb phiB, sum phiA)

@@ -40,12 +40,19 @@ struct scft_ctx {
     // reductions / small arrays
     double* dpart = nullptr;
     int nblk = 0;
-    double *dsm = nullptr, *dlam = nullptr, *dx = nullptr, *dxref = nullptr, *dcproj = nullptr, *dphisv = nullptr;
+    double *dsm = nullptr, *dlam = nullptr, *dcproj = nullptr, *dphisv = nullptr;
     int *dactive = nullptr, *dslot = nullptr;
     // pinned host staging
     double* hsm = nullptr;  // [R*SCAL_N]
     int* hslot = nullptr;   // [R]
     double* hsmall = nullptr;  // [4R] x, xref, lam, cproj
+    // spline weights of one launch (spline_weights): device [R][R] and pinned staging
+    double *dG = nullptr, *hG = nullptr;
+    // chunked host-row spline calls (density.h:462-528): staging of [R][spl_chunk] grid
+    // points (device in / out, pinned host), nodes and evaluation points
+    int spl_chunk = 0;
+    double *dspl_in = nullptr, *dspl_out = nullptr, *hspl = nullptr;
+    std::vector<double> spl_x, spl_xref;
     int* hact = nullptr;    // [R]
     // Anderson mixing (opt-in, dostring = 0): per replica rings of the last H+1 (W, DEV)
     // pairs [R][H+1][2M], the Gram sums of their DEVs on the host, and small staging
@@ -440,8 +447,6 @@ scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
               hipMalloc((void**)&c->dpart, sizeof(double) * (size_t)c->nblk * 4 * R) == hipSuccess &&
               hipMalloc((void**)&c->dsm, sizeof(double) * SCAL_N * R) == hipSuccess &&
               hipMalloc((void**)&c->dlam, sizeof(double) * R) == hipSuccess &&
-              hipMalloc((void**)&c->dx, sizeof(double) * R) == hipSuccess &&
-              hipMalloc((void**)&c->dxref, sizeof(double) * R) == hipSuccess &&
               hipMalloc((void**)&c->dcproj, sizeof(double) * R) == hipSuccess &&
               hipMalloc((void**)&c->dphisv, sizeof(double) * 5 * R) == hipSuccess &&
               hipMalloc((void**)&c->dactive, sizeof(int) * R) == hipSuccess &&
@@ -449,7 +454,10 @@ scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
               hipHostMalloc((void**)&c->hsm, sizeof(double) * SCAL_N * R, hipHostMallocDefault) == hipSuccess &&
               hipHostMalloc((void**)&c->hslot, sizeof(int) * R, hipHostMallocDefault) == hipSuccess &&
               hipHostMalloc((void**)&c->hsmall, sizeof(double) * 4 * R, hipHostMallocDefault) == hipSuccess &&
-              hipHostMalloc((void**)&c->hact, sizeof(int) * R, hipHostMallocDefault) == hipSuccess;
+              hipHostMalloc((void**)&c->hact, sizeof(int) * R, hipHostMallocDefault) == hipSuccess &&
+              hipMalloc((void**)&c->dG, sizeof(double) * R * R) == hipSuccess &&
+              hipHostMalloc((void**)&c->hG, sizeof(double) * R * R, hipHostMallocDefault) == hipSuccess &&
+              scft::row_mix_setup() == hipSuccess;
     if (ok && p->anderson > 0) {
         const size_t L = (size_t)p->anderson + 1;
         ok = hipMalloc((void**)&c->dAW, sizeof(double) * L * 2 * M * R) == hipSuccess &&
@@ -488,11 +496,13 @@ scft_status scft_ctx_destroy(scft_ctx* c)
     c->pe64.reset();
     c->pe64_fe.reset();
     void* dptrs[] = {c->dW,    c->dphi,  c->dDEV,   c->ddWda,   c->dWs,    c->dphis_rows, c->dpart,
-                     c->dsm,   c->dlam,  c->dx,     c->dxref,   c->dcproj, c->dphisv,     c->dactive,
-                     c->dslot, c->dAW,   c->dAD,    c->dApart,  c->dAcoef, c->dArows};
+                     c->dsm,   c->dlam,  c->dcproj, c->dphisv,     c->dactive,
+                     c->dslot, c->dAW,   c->dAD,    c->dApart,  c->dAcoef, c->dArows, c->dG};
     for (void* q : dptrs)
         if (q) (void)hipFree(q);
-    void* hptrs[] = {c->hsm, c->hslot, c->hsmall, c->hact, c->hAbuf, c->hArows};
+    void* hptrs[] = {c->hsm, c->hslot, c->hsmall, c->hact, c->hAbuf, c->hArows, c->hG, c->hspl};
+    for (void* q : {(void*)c->dspl_in, (void*)c->dspl_out})
+        if (q) (void)hipFree(q);
     for (void* q : hptrs)
         if (q) (void)hipHostFree(q);
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -622,6 +632,183 @@ scft_status scft_solve_w_plus(scft_ctx* c, int maxIter, double errTol, const int
     return SCFT_OK;
 }
 
+// Weights of the natural cubic spline through (x[i], a[i]), i < R (cubefit_chunk_g,
+// GPU_kernels.h:232-284), evaluated at the points xe[erows[j]], j < ne: G[j][i] is the
+// derivative (deriv) or the value there per unit a[i]. A point lies on the interval of the
+// last node strictly below it (GPU_kernels.h:291-297, 313-319). R >= 2.
+static void spline_weights(int R, const double* x, const double* xe, const int* erows, int ne, bool deriv, double* G)
+{
+    const int n = R - 1;
+    std::vector<double> h(R, 0.0), l(R, 0.0), u(R, 0.0), z(R, 0.0), c(R, 0.0);
+    for (int i = 0; i < n; ++i) h[i] = x[i + 1] - x[i];
+    l[0] = 1.0;
+    for (int i = 1; i < n; ++i) {
+        l[i] = 2.0 * (x[i + 1] - x[i - 1]) - h[i - 1] * u[i - 1];
+        u[i] = h[i] / l[i];
+    }
+    std::vector<int> nm(ne);
+    std::vector<double> dx(ne);
+    for (int j = 0; j < ne; ++j) {
+        const double xx = xe[erows[j]];
+        int k = 0;
+        for (int i = 0; i < R; ++i) {
+            if (x[i] < xx) k = i;
+            else break;
+        }
+        k = std::max(0, std::min(k, n - 1));  // n - 1 bounds only an x that is not increasing
+        nm[j] = k;
+        dx[j] = xx - x[k];
+    }
+    for (int col = 0; col < R; ++col) {
+        auto a = [col](int i) { return i == col ? 1.0 : 0.0; };
+        z[0] = 0.0;
+        for (int i = 1; i < n; ++i) {
+            const double A = 3.0 * (a(i + 1) - a(i)) / h[i] - 3.0 * (a(i) - a(i - 1)) / h[i - 1];
+            z[i] = (A - h[i - 1] * z[i - 1]) / l[i];
+        }
+        c[n] = 0.0;
+        for (int j = n - 1; j >= 0; --j) c[j] = z[j] - u[j] * c[j + 1];
+        for (int j = 0; j < ne; ++j) {
+            const int k = nm[j];
+            const double t = dx[j];
+            const double b = (a(k + 1) - a(k)) / h[k] - h[k] * (c[k + 1] + 2.0 * c[k]) / 3.0;
+            const double d = (c[k + 1] - c[k]) / (3.0 * h[k]);
+            G[(size_t)j * R + col] = deriv ? b + 2.0 * t * c[k] + 3.0 * t * t * d
+                                           : a(k) + t * b + t * t * c[k] + t * t * t * d;
+        }
+    }
+}
+
+scft_status scft_spline_weights(int nrep, const double* x, const double* xe, const int* erows, int ne, int deriv,
+                                double* G)
+{
+    if (nrep < 2 || nrep > scft::SPLINE_RMAX || !x || !xe || !erows || ne < 0 || ne > nrep || !G)
+        return SCFT_ERR_INVALID;
+    for (int i = 0; i + 1 < nrep; ++i)
+        if (!(x[i + 1] > x[i])) return SCFT_ERR_INVALID;
+    for (int j = 0; j < ne; ++j)
+        if (erows[j] < 0 || erows[j] >= nrep) return SCFT_ERR_INVALID;
+    spline_weights(nrep, x, xe, erows, ne, deriv != 0, G);
+    return SCFT_OK;
+}
+
+// The tangent dW-/dalpha at the nodes (doperp, scft.cu:643-650) of the replicas in rows, or the
+// redistribution W- += lambda (spline(xref) - W-) of them (doredist, scft.cu:745-757,
+// density.h:511-528): one weight matrix, one launch over the rows W [R][stride] (W- part,
+// first M of each row). tangent: out rows of stride ostride; redistribution: in place.
+static scft_status spline_launch(scft_ctx* c, const double* x, const double* xref, const std::vector<int>& rows,
+                          bool tangent, double lambda, const double* in, long long istride, double* out,
+                          long long ostride, long long M)
+{
+    const int R = c->R, ne = (int)rows.size();
+    if (ne == 0) return SCFT_OK;
+    // the staging buffers of an earlier launch may still be in flight
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int j = 0; j < ne; ++j) c->hact[j] = rows[j];
+    spline_weights(R, x, tangent ? x : xref, rows.data(), ne, tangent, c->hG);
+    if (!tangent)
+        for (int j = 0; j < ne; ++j)
+            for (int i = 0; i < R; ++i)
+                c->hG[(size_t)j * R + i] = lambda * c->hG[(size_t)j * R + i] + (i == rows[j] ? 1.0 - lambda : 0.0);
+    HIPCHK(c, hipMemcpyAsync(c->dG, c->hG, sizeof(double) * ne * R, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->dactive, c->hact, sizeof(int) * ne, hipMemcpyHostToDevice, c->st));
+    scft::launch_row_mix(R, M, c->dG, c->dactive, ne, in, istride, out, ostride, c->st);
+    HIPCHK(c, hipGetLastError());
+    return SCFT_OK;
+}
+
+// ---------------------------------------------------------------- chunked host-row spline
+scft_status scft_spline_setup(scft_ctx* c, int chunkM)
+{
+    if (!c || chunkM <= 0) return fail(c, SCFT_ERR_INVALID, "scft_spline_setup: chunkM must be positive");
+    SCFTCHK(bind_device(c));
+    const int R = c->R;
+    const long long n = std::min<long long>(chunkM, c->M);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (void* q : {(void*)c->dspl_in, (void*)c->dspl_out})
+        if (q) (void)hipFree(q);
+    if (c->hspl) (void)hipHostFree(c->hspl);
+    c->dspl_in = c->dspl_out = c->hspl = nullptr;
+    c->spl_chunk = 0;
+    const size_t bytes = sizeof(double) * (size_t)R * n;
+    if (hipMalloc((void**)&c->dspl_in, bytes) != hipSuccess || hipMalloc((void**)&c->dspl_out, bytes) != hipSuccess ||
+        hipHostMalloc((void**)&c->hspl, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, SCFT_ERR_NOMEM, "scft_spline_setup: staging allocation failed");
+    }
+    c->spl_chunk = (int)n;
+    return SCFT_OK;
+}
+
+scft_status scft_spline_set_x(scft_ctx* c, const double* x)
+{
+    if (!c || !x) return fail(c, SCFT_ERR_INVALID, "scft_spline_set_x: null");
+    for (int i = 0; i + 1 < c->R; ++i)
+        if (!(x[i + 1] > x[i])) return fail(c, SCFT_ERR_INVALID, "scft_spline_set_x: x must increase strictly");
+    c->spl_x.assign(x, x + c->R);
+    return SCFT_OK;
+}
+
+scft_status scft_spline_set_xref(scft_ctx* c, const double* xref)
+{
+    if (!c || !xref) return fail(c, SCFT_ERR_INVALID, "scft_spline_set_xref: null");
+    c->spl_xref.assign(xref, xref + c->R);
+    return SCFT_OK;
+}
+
+// Checks a chunk call and copies rows ww[ii][r0 .. r0+n) into the device staging [R][n].
+static scft_status spline_chunk_in(scft_ctx* c, const double* const* ww, int r0, int n, const int* doiis,
+                                   bool need_xref, const char* who)
+{
+    if (!c || !ww || !doiis) return fail(c, SCFT_ERR_INVALID, std::string(who) + ": null");
+    if (c->R < 2) return fail(c, SCFT_ERR_INVALID, std::string(who) + ": the spline needs at least two replicas");
+    if (c->spl_chunk == 0) return fail(c, SCFT_ERR_INVALID, std::string(who) + ": call scft_spline_setup first");
+    if ((int)c->spl_x.size() != c->R || (need_xref && (int)c->spl_xref.size() != c->R))
+        return fail(c, SCFT_ERR_INVALID, std::string(who) + ": nodes not set (scft_spline_set_x / _xref)");
+    if (r0 < 0 || n <= 0 || n > c->spl_chunk || (long long)r0 + n > c->M)
+        return fail(c, SCFT_ERR_INVALID, std::string(who) + ": chunk outside the grid or the staging");
+    SCFTCHK(bind_device(c));
+    HIPCHK(c, hipStreamSynchronize(c->st));  // the staging of an earlier call is free
+    for (int ii = 0; ii < c->R; ++ii) {
+        if (!ww[ii]) return fail(c, SCFT_ERR_INVALID, std::string(who) + ": null row");
+        std::memcpy(c->hspl + (size_t)ii * n, ww[ii] + r0, sizeof(double) * n);
+    }
+    HIPCHK(c, hipMemcpyAsync(c->dspl_in, c->hspl, sizeof(double) * (size_t)c->R * n, hipMemcpyHostToDevice, c->st));
+    return SCFT_OK;
+}
+
+scft_status scft_spline_derivs(scft_ctx* c, const double* const* ww, int r0, int n, const int* doiis,
+                               double* const* out)
+{
+    SCFTCHK(spline_chunk_in(c, ww, r0, n, doiis, false, "scft_spline_derivs"));
+    if (!out) return fail(c, SCFT_ERR_INVALID, "scft_spline_derivs: null output");
+    std::vector<int> rows;
+    for (int ii = 0; ii < c->R; ++ii)
+        if (doiis[ii] == 1) {
+            if (!out[ii]) return fail(c, SCFT_ERR_INVALID, "scft_spline_derivs: null output row");
+            rows.push_back(ii);
+        }
+    SCFTCHK(spline_launch(c, c->spl_x.data(), nullptr, rows, true, 0.0, c->dspl_in, n, c->dspl_out, n, n));
+    HIPCHK(c, hipMemcpyAsync(c->hspl, c->dspl_out, sizeof(double) * (size_t)c->R * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int ii : rows) std::memcpy(out[ii] + r0, c->hspl + (size_t)ii * n, sizeof(double) * n);
+    return SCFT_OK;
+}
+
+scft_status scft_spline_redist(scft_ctx* c, double* const* ww, int r0, int n, const int* doiis, double lambda)
+{
+    SCFTCHK(spline_chunk_in(c, ww, r0, n, doiis, true, "scft_spline_redist"));
+    std::vector<int> rows;
+    for (int ii = 1; ii < c->R - 1; ++ii)
+        if (doiis[ii] == 1) rows.push_back(ii);
+    SCFTCHK(spline_launch(c, c->spl_x.data(), c->spl_xref.data(), rows, false, lambda, c->dspl_in, n, c->dspl_in, n,
+                          n));
+    HIPCHK(c, hipMemcpyAsync(c->hspl, c->dspl_in, sizeof(double) * (size_t)c->R * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int ii : rows) std::memcpy(ww[ii] + r0, c->hspl + (size_t)ii * n, sizeof(double) * n);
+    return SCFT_OK;
+}
+
 // one solve_field iteration (scft.cu:592-757), single process, doii = -1
 scft_status scft_string_iterate(scft_ctx* c, int k, double errTol, double* alf, scft_iter_report* rep)
 {
@@ -672,14 +859,7 @@ scft_status scft_string_iterate(scft_ctx* c, int k, double errTol, double* alf, 
     std::vector<int> proj(R, 0);  // interior replicas whose DEV- is projected (scft.cu:670-678)
     for (int ii = 0; ii < R; ++ii) c->hsmall[3 * R + ii] = 0.0;  // cproj
     if (doperp == 1) {
-        for (int ii = 0; ii < R; ++ii) {
-            c->hsmall[ii] = x[ii];
-            c->hact[ii] = doiis[ii];
-        }
-        HIPCHK(c, hipMemcpyAsync(c->dx, c->hsmall, sizeof(double) * R, hipMemcpyHostToDevice, c->st));
-        HIPCHK(c, hipMemcpyAsync(c->dactive, c->hact, sizeof(int) * R, hipMemcpyHostToDevice, c->st));
-        scft::launch_spline_deriv(R, M, c->dx, c->dactive, c->dW, c->ddWda, c->st);
-        HIPCHK(c, hipGetLastError());
+        SCFTCHK(spline_launch(c, x.data(), nullptr, active, true, 0.0, c->dW, 2 * M, c->ddWda, M, M));
         scft::launch_tangent_dots(c->dslot, nb, M, c->ddWda, c->dDEV, c->dpart, c->nblk, c->st);
         HIPCHK(c, hipGetLastError());
         std::vector<double> td;
@@ -730,16 +910,10 @@ scft_status scft_string_iterate(scft_ctx* c, int k, double errTol, double* alf, 
         for (int ii = 0; ii < R; ++ii) alf[ii] = x[ii];
 
     if (doredist == 1) {
-        for (int ii = 0; ii < R; ++ii) {
-            c->hsmall[ii] = x[ii];
-            c->hsmall[R + ii] = xref[ii];
-            c->hact[ii] = doiis[ii];
-        }
-        HIPCHK(c, hipMemcpyAsync(c->dx, c->hsmall, sizeof(double) * R, hipMemcpyHostToDevice, c->st));
-        HIPCHK(c, hipMemcpyAsync(c->dxref, c->hsmall + R, sizeof(double) * R, hipMemcpyHostToDevice, c->st));
-        HIPCHK(c, hipMemcpyAsync(c->dactive, c->hact, sizeof(int) * R, hipMemcpyHostToDevice, c->st));
-        scft::launch_redistribute(R, M, c->dx, c->dxref, c->dactive, lambda, c->dW, c->st);
-        HIPCHK(c, hipGetLastError());
+        std::vector<int> inner;  // interior replicas that move (density.h:524-527)
+        for (int ii = 1; ii < R - 1; ++ii)
+            if (doiis[ii]) inner.push_back(ii);
+        SCFTCHK(spline_launch(c, x.data(), xref.data(), inner, false, lambda, c->dW, 2 * M, c->dW, 2 * M, M));
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
 
