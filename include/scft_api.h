@@ -90,6 +90,8 @@ typedef struct scft_ctx scft_ctx;
  * memory for nrep replicas. */
 scft_status scft_ctx_create(const scft_params *p, scft_ctx **out);
 scft_status scft_ctx_destroy(scft_ctx *ctx);
+/* Message of the last failed call on ctx; with ctx NULL, of the calling thread's last failed
+ * call that takes no context (scft_spline_derivs_noalloc). */
 const char *scft_last_error(const scft_ctx *ctx);
 scft_status scft_sync(scft_ctx *ctx);
 /* FFT kernels the propagator runs on: "two-stage" or "stockham" (plane/column kernels
@@ -221,6 +223,24 @@ scft_status scft_spline_derivs(scft_ctx *ctx, const double *const *ww, int r0, i
  * with s the spline through (x[j], ww[j][r]), for interior ii (0 < ii < nrep - 1) with
  * doiis[ii] == 1 and r in the chunk; the spline is fitted to the values before the update. */
 scft_status scft_spline_redist(scft_ctx *ctx, double *const *ww, int r0, int n, const int *doiis, double lambda);
+
+/* No counterpart in the reference. For callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc): the spline
+ * tangent of scft_spline_derivs (splineDerivsChunked, density.h:491-509) on device rows and
+ * without a context. dout[ii][r] = derivative at x[ii] of the natural cubic spline through
+ * (x[j], dW[j][r]), for every ii with doiis[ii] == 1 and r < M; other rows of dout are not
+ * touched. dW: nrep device rows of stride wstride >= M (W- first); dout: device rows of
+ * stride ostride >= M; x (strictly increasing) and doiis: host arrays of nrep.
+ * The work is queued on `stream` (a hipStream_t; NULL: the null stream) and not waited for.
+ * Besides dW and dout it uses only a static device array of the library (33 KB per GPU,
+ * held until the process ends): nrep above 64 is refused with SCFT_ERR_UNSUPPORTED. Every
+ * call on one GPU shares that array and rewrites all of it, so calls that overlap in time on
+ * one GPU (on other streams or threads) race; calls on one stream are ordered. It allocates
+ * nothing and makes no host-device copy. scft_last_error(NULL) gives the calling thread's
+ * last message. */
+scft_status scft_spline_derivs_noalloc(int nrep, long long M, const double *x, const int *doiis,
+                                       const double *dW, long long wstride, double *dout,
+                                       long long ostride, void *stream);
 
 /* Weights of the natural cubic spline through (x[i], a[i]), i < nrep (cubefit_chunk_g,
  * GPU_kernels.h:232-284), at the points xe[erows[j]], j < ne: G[j][i] (row-major, ne x nrep)

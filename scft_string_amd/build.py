@@ -24,7 +24,7 @@ ARCH = os.environ.get("SCFT_OFFLOAD_ARCH", "gfx950")
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature",
           "-Xclang", "-packed-fp32-ops", "-I", INC, "-I", CSRC, "-Wall", "-Wno-unused-result"]
 
-SOURCES = ["prop_engine.hip", "field_kernels.hip", "scft_api.hip"]
+SOURCES = ["prop_engine.hip", "field_kernels.hip", "scft_api.hip", "scft_noalloc.hip"]
 
 
 def _compile(src):

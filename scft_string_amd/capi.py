@@ -22,7 +22,7 @@ EXPORTED = [
     "scft_debug_qq_invariant", "scft_debug_time_props", "scft_fft_path", "scft_dlnq_dbox",
     "scft_box_stress", "scft_set_box", "scft_relax_box", "scft_iterate_concurrent", "scft_engine_info",
     "scft_backward", "scft_spline_setup", "scft_spline_set_x", "scft_spline_set_xref", "scft_spline_derivs",
-    "scft_spline_redist", "scft_spline_weights",
+    "scft_spline_redist", "scft_spline_weights", "scft_spline_derivs_noalloc",
 ]
 _STRING_RESULT = ("scft_last_error", "scft_fft_path", "scft_engine_info")
 
@@ -96,6 +96,8 @@ def lib():
         L.scft_spline_derivs.argtypes = [vp, C.POINTER(dp), C.c_int, C.c_int, ip, C.POINTER(dp)]
         L.scft_spline_redist.argtypes = [vp, C.POINTER(dp), C.c_int, C.c_int, ip, C.c_double]
         L.scft_spline_weights.argtypes = [C.c_int, dp, dp, ip, C.c_int, C.c_int, dp]
+        L.scft_spline_derivs_noalloc.argtypes = [C.c_int, C.c_longlong, dp, ip, C.c_void_p, C.c_longlong,
+                                                 C.c_void_p, C.c_longlong, C.c_void_p]
         for name in EXPORTED:
             if name not in _STRING_RESULT:
                 getattr(L, name).restype = C.c_int

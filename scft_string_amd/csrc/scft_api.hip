@@ -356,7 +356,7 @@ scft_status anderson_iterate(scft_ctx* c, double* alf, scft_iter_report* rep)
 
 extern "C" {
 
-const char* scft_last_error(const scft_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* scft_last_error(const scft_ctx* ctx) { return ctx ? ctx->err.c_str() : scft::noctx_error(); }
 
 const char* scft_fft_path(const scft_ctx* ctx)
 {
@@ -632,51 +632,11 @@ scft_status scft_solve_w_plus(scft_ctx* c, int maxIter, double errTol, const int
     return SCFT_OK;
 }
 
-// Weights of the natural cubic spline through (x[i], a[i]), i < R (cubefit_chunk_g,
-// GPU_kernels.h:232-284), evaluated at the points xe[erows[j]], j < ne: G[j][i] is the
-// derivative (deriv) or the value there per unit a[i]. A point lies on the interval of the
-// last node strictly below it (GPU_kernels.h:291-297, 313-319). R >= 2.
+// Weights of the natural cubic spline at the points xe[erows[j]], j < ne (spline_weight_row):
+// G [ne][R] row-major. R >= 2.
 static void spline_weights(int R, const double* x, const double* xe, const int* erows, int ne, bool deriv, double* G)
 {
-    const int n = R - 1;
-    std::vector<double> h(R, 0.0), l(R, 0.0), u(R, 0.0), z(R, 0.0), c(R, 0.0);
-    for (int i = 0; i < n; ++i) h[i] = x[i + 1] - x[i];
-    l[0] = 1.0;
-    for (int i = 1; i < n; ++i) {
-        l[i] = 2.0 * (x[i + 1] - x[i - 1]) - h[i - 1] * u[i - 1];
-        u[i] = h[i] / l[i];
-    }
-    std::vector<int> nm(ne);
-    std::vector<double> dx(ne);
-    for (int j = 0; j < ne; ++j) {
-        const double xx = xe[erows[j]];
-        int k = 0;
-        for (int i = 0; i < R; ++i) {
-            if (x[i] < xx) k = i;
-            else break;
-        }
-        k = std::max(0, std::min(k, n - 1));  // n - 1 bounds only an x that is not increasing
-        nm[j] = k;
-        dx[j] = xx - x[k];
-    }
-    for (int col = 0; col < R; ++col) {
-        auto a = [col](int i) { return i == col ? 1.0 : 0.0; };
-        z[0] = 0.0;
-        for (int i = 1; i < n; ++i) {
-            const double A = 3.0 * (a(i + 1) - a(i)) / h[i] - 3.0 * (a(i) - a(i - 1)) / h[i - 1];
-            z[i] = (A - h[i - 1] * z[i - 1]) / l[i];
-        }
-        c[n] = 0.0;
-        for (int j = n - 1; j >= 0; --j) c[j] = z[j] - u[j] * c[j + 1];
-        for (int j = 0; j < ne; ++j) {
-            const int k = nm[j];
-            const double t = dx[j];
-            const double b = (a(k + 1) - a(k)) / h[k] - h[k] * (c[k + 1] + 2.0 * c[k]) / 3.0;
-            const double d = (c[k + 1] - c[k]) / (3.0 * h[k]);
-            G[(size_t)j * R + col] = deriv ? b + 2.0 * t * c[k] + 3.0 * t * t * d
-                                           : a(k) + t * b + t * t * c[k] + t * t * t * d;
-        }
-    }
+    for (int j = 0; j < ne; ++j) scft::spline_weight_row(R, x, xe[erows[j]], deriv, G + (size_t)j * R);
 }
 
 scft_status scft_spline_weights(int nrep, const double* x, const double* xe, const int* erows, int ne, int deriv,
