@@ -61,7 +61,8 @@ def cpu_baseline(m, D, N, R_active, props_per_iter, budget_s=20.0):
     Also one props on a single thread, reported beside it."""
     from oracle import orc
 
-    threads = min(os.cpu_count() or 1, 16)
+    ncpu = os.cpu_count() or 1
+    threads = min(ncpu, 16)   # the CPU share of one GPU on the measurement hosts
     orc.set_threads(threads)
     g = orc.Grid(m, D)
     W = initial_fields(m, D, 3)[1]
@@ -78,8 +79,14 @@ def cpu_baseline(m, D, N, R_active, props_per_iter, budget_s=20.0):
     g.props(W, N=N, alpha=CFG["alpha"], phic=CFG["phidb"], f=CFG["f"], ens=CFG["ens"])
     dt1 = time.perf_counter() - t1
     orc.set_threads(threads)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((l.split(":", 1)[1].strip() for l in fh if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": 1.0 / (dt * R_active * props_per_iter), "unit": "string iters/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host": {"nproc": ncpu, "cpu": model, "threads_used": threads},
             "sample": f"{n} FP64 props of one replica at {m[0]}^3, N={N} ({dt:.2f} s each), scaled to "
                       f"{R_active} replicas x {props_per_iter} props per iteration",
             "single_thread": {"value": 1.0 / (dt1 * R_active * props_per_iter), "cores": 1,
@@ -228,6 +235,61 @@ def pmc_probe():
     print("pmc probe done", flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_contour_step_14x64cube_N128_fp32.json")
+CONTOUR_KERNELS = ("k_column2<float, 64, 64, 64, 64>", "k_plane_mid2<float, 64, 64>", "k_plane_end2<float, 64, 64>")
+
+
+def measure_traffic(N, timeout=150):
+    """Memory traffic of one contour step of the benchmark batch, measured now: two child
+    runs of `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` over `bench.py --pmc-probe`
+    (a 512 MiB copy, then one props of the 14 active replicas). Each counter is scaled by what
+    it reads for the copy (512 MiB read and written), and the contour kernels' dispatches
+    are summed and divided by N. Returns (bytes per step, details) or (None, reason)."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, "rocprofv3 not found"
+    tot = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            out = os.path.join(td, ctr)
+            cmd = [exe, "--pmc", ctr, "--output-format", "csv", "-d", out, "-o", "pmc", "--", sys.executable,
+                   os.path.join(ROOT, "bench.py"), "--pmc-probe"]
+            try:
+                r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), capture_output=True,
+                                   text=True, timeout=timeout)
+            except subprocess.TimeoutExpired:
+                return None, f"rocprofv3 --pmc {ctr} timed out"
+            files = [os.path.join(dp, f) for dp, _, fs in os.walk(out) for f in fs if f.endswith("counter_collection.csv")]
+            if r.returncode != 0 or not files:
+                return None, f"rocprofv3 --pmc {ctr} failed (rc {r.returncode})"
+            copy_kib, kern_kib, disp = None, 0.0, 0
+            for row in csv.DictReader(open(files[0])):
+                if row["Counter_Name"] != ctr:
+                    continue
+                name, v = row["Kernel_Name"], float(row["Counter_Value"])
+                if name.startswith("__amd_rocclr_copyBuffer") and int(row["Grid_Size"]) >= 1 << 16:
+                    copy_kib = v
+                elif any(k in name for k in CONTOUR_KERNELS):
+                    kern_kib += v
+                    disp += 1
+            if not copy_kib:
+                return None, f"no copy kernel in the {ctr} run"
+            scale = (512 << 20) / (copy_kib * 1024.0)
+            tot[ctr] = (kern_kib * 1024.0 * scale / N, scale, disp)
+    byte = tot["FETCH_SIZE"][0] + tot["WRITE_SIZE"][0]
+    return byte, {"source": "measured in this run: rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE over bench.py --pmc-probe",
+                  "read_bytes_per_step": tot["FETCH_SIZE"][0], "write_bytes_per_step": tot["WRITE_SIZE"][0],
+                  "scale_fetch": tot["FETCH_SIZE"][1], "scale_write": tot["WRITE_SIZE"][1],
+                  "dispatches": tot["FETCH_SIZE"][2],
+                  "note": "L2-to-memory bytes; hits in the 256 MB Infinity Cache beyond L2 are not separated by "
+                          "these counters (TCC_EA0_*_DRAM_32B read the same)"}
+
+
 def batched_fp64_rate(W0, warmup, steps):
     """The same string with an FP64 propagator (every replica batched on the GPU): string
     iterations per second over `steps` iterations after `warmup`, and the contour step."""
@@ -298,6 +360,64 @@ def adjoint_report():
     print(json.dumps(out), flush=True)
 
 
+C4 = dict(chi=22.0, f=0.4, phidb=0.85, ens=2, alpha=0.1)
+
+
+def uneven_solves():
+    """The 12 independent solves of the uneven batch, {(m, N): count}: m in {48^3, 64^3, 96^3}
+    and N in {64, 100, 128} drawn with seed 4."""
+    rng0 = np.random.default_rng(4)
+    solves = {}
+    for _ in range(12):
+        m = [(48,) * 3, (64,) * 3, (96,) * 3][rng0.integers(3)]
+        N = [64, 100, 128][rng0.integers(3)]
+        solves[(m, N)] = solves.get((m, N), 0) + 1
+    return solves
+
+
+def uneven_specs(with_string, precision="fp32"):
+    """Every context of the uneven batch as (Context keyword arguments, initial W [R][2][M]):
+    the 24-replica 128^3, N = 160 grand-canonical string (chiN = 22, f = 0.4, phidb = 0.85,
+    lamellar -> gyroid-like ends, D = 8) when with_string, then one context per distinct
+    (m, N) of the 12 solves, its solves as independent replicas (dostring = 0, D = m/16)."""
+    def grid(m):
+        return np.meshgrid(*[2 * np.pi * np.arange(m[i]) / m[i] for i in range(3)], indexing="ij")
+
+    rng = np.random.default_rng(7)
+    specs = []
+    if with_string:
+        m, R = (128,) * 3, 24
+        X, Y, Z = grid(m)
+        lam = 0.5 * np.cos(X)
+        gyr = 0.5 * (np.sin(X) * np.cos(Y) + np.sin(Y) * np.cos(Z) + np.sin(Z) * np.cos(X))
+        W = np.zeros((R, 2, X.size))
+        for i in range(R):
+            t = i / (R - 1)
+            W[i, 0] = ((1 - t) * lam + t * gyr).reshape(-1) + 0.01 * rng.standard_normal(X.size)
+        specs.append((dict(m=m, D=(8.0,) * 3, N=160, nrep=R, precision=precision, fix=(1, 1), dostring=1, **C4), W))
+    for (m, N), R in sorted(uneven_solves().items()):
+        X = grid(m)[0]
+        W = np.zeros((R, 2, X.size))
+        for i in range(R):
+            W[i, 0] = (0.5 * np.cos(X)).reshape(-1) + 0.01 * rng.standard_normal(X.size)
+        specs.append((dict(m=m, D=(m[0] / 16.0,) * 3, N=N, nrep=R, precision=precision, fix=(0, 0), dostring=0, **C4),
+                      W))
+    return specs
+
+
+def uneven_contexts(with_string, precision="fp32"):
+    from scft_string_amd import Context
+
+    ctxs = []
+    for kw, W in uneven_specs(with_string, precision):
+        kw = dict(kw)
+        c = Context(kw.pop("m"), kw.pop("D"), kw.pop("N"), kw.pop("f"), kw.pop("alpha"), kw.pop("ens"),
+                    kw.pop("phidb"), kw.pop("chi"), **kw)
+        c.set_fields(W)
+        ctxs.append(c)
+    return ctxs
+
+
 def uneven_report(K):
     """Uneven solves on one GPU: a 24-replica string at 128^3, N = 160, FP32, grand canonical
     (chiN = 22, f = 0.4, phidb = 0.85), lamellar -> gyroid-like ends, plus 12 independent
@@ -307,43 +427,12 @@ def uneven_report(K):
     sequential rounds iterate the contexts one after another from identical fresh states."""
     import gc
 
-    from scft_string_amd import Context, iterate_concurrent
+    from scft_string_amd import iterate_concurrent
 
-    chi, f, phidb, ens, alpha = 22.0, 0.4, 0.85, 2, 0.1
-    rng0 = np.random.default_rng(4)
-    solves = {}
-    for _ in range(12):
-        m = [(48,) * 3, (64,) * 3, (96,) * 3][rng0.integers(3)]
-        N = [64, 100, 128][rng0.integers(3)]
-        solves[(m, N)] = solves.get((m, N), 0) + 1
-
-    def grid(m):
-        return np.meshgrid(*[2 * np.pi * np.arange(m[i]) / m[i] for i in range(3)], indexing="ij")
+    solves = uneven_solves()
 
     def build(with_string):
-        rng = np.random.default_rng(7)
-        ctxs = []
-        if with_string:
-            m, R = (128,) * 3, 24
-            X, Y, Z = grid(m)
-            lam = 0.5 * np.cos(X)
-            gyr = 0.5 * (np.sin(X) * np.cos(Y) + np.sin(Y) * np.cos(Z) + np.sin(Z) * np.cos(X))
-            W = np.zeros((R, 2, X.size))
-            for i in range(R):
-                t = i / (R - 1)
-                W[i, 0] = ((1 - t) * lam + t * gyr).reshape(-1) + 0.01 * rng.standard_normal(X.size)
-            c = Context(m, (8.0,) * 3, 160, f, alpha, ens, phidb, chi, nrep=R, precision="fp32", fix=(1, 1))
-            c.set_fields(W)
-            ctxs.append(c)
-        for (m, N), R in sorted(solves.items()):
-            X = grid(m)[0]
-            W = np.zeros((R, 2, X.size))
-            for i in range(R):
-                W[i, 0] = (0.5 * np.cos(X)).reshape(-1) + 0.01 * rng.standard_normal(X.size)
-            c = Context(m, (m[0] / 16.0,) * 3, N, f, alpha, ens, phidb, chi, nrep=R, precision="fp32", dostring=0)
-            c.set_fields(W)
-            ctxs.append(c)
-        return ctxs
+        return uneven_contexts(with_string)
 
     def run(with_string, concurrent):
         ctxs = build(with_string)
@@ -387,6 +476,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the batched FP64 rate of the same string")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--pmc-probe", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic measurement")
     ap.add_argument("--adjoint", action="store_true",
                     help="box-gradient timing: 64^3, N=100, FP32, D swept over 5.6-6.4, 1 and 8 replicas")
     ap.add_argument("--uneven", action="store_true",
@@ -456,10 +546,13 @@ def main():
     engine = ctx.engine_info.split(";")[0]
     ctx.close()
 
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "pmc_traffic_contour_step_14x64cube_N128_fp32.json")
-    if a.precision == "fp32" and os.path.exists(tf):
-        traffic = json.load(open(tf)).get("hbm_bytes_per_contour_step")
+    traffic, traffic_info = None, None
+    if a.precision == "fp32" and not a.no_traffic:
+        traffic, traffic_info = measure_traffic(c["N"])
+        if traffic is None and os.path.exists(TRAFFIC_PROFILE):   # fall back to the committed profile
+            why = traffic_info
+            traffic = json.load(open(TRAFFIC_PROFILE)).get("hbm_bytes_per_contour_step")
+            traffic_info = {"source": f"{os.path.relpath(TRAFFIC_PROFILE, ROOT)} (not measured in this run: {why})"}
 
     out = {
         "metric": "string iters/sec 16x64^3xN128 (batched FP32 propagator, FP64 field update)",
@@ -474,7 +567,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "contour step (column, plane_mid, column, plane_end) over "
                                f"{n_act} replicas", "ms_per_contour_step": ms_step,
-                     "algorithmic_bytes_per_contour_step": bstep},
+                     "algorithmic_bytes_per_contour_step": bstep, "traffic_info": traffic_info,
+                     "time_base": "HIP events on the context stream around the contour loop of props calls "
+                                  "(two replica-group streams overlapping); the one-stream kernel-trace averages "
+                                  "are in profiles/kernel_trace_string_16x64cube_N128_fp32.md"},
     }
     if a.precision == "fp32" and not a.no_fp64:
         out["batched_fp64"] = batched_fp64_rate(W0, a.warmup, 2)

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace scft {
@@ -317,7 +318,7 @@ template <typename T>
 PropEngine<T>::~PropEngine()
 {
     void* ptrs[] = {Ch_,   Cf_,   e4A_,  e4B_,  qs_, tw_[0], tw_[1], tw_[2], g_[0],
-                    g_[1], g_[2], part_, cAB_, scal_, K0_, K1_,    Kf_,    gpart_};
+                    g_[1], g_[2], part_, cAB_, scal_, K0_, K1_,    Kf_,    gpart_, rn_};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : ev_)
@@ -457,7 +458,8 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
         !alloc((void**)&e4B_, sizeof(T) * (size_t)M_ * nbmax) ||
         !alloc((void**)&qs_, sizeof(cpx<T>) * (size_t)M_ * (size_t)qpitch_ * nbmax) ||
         !alloc((void**)&part_, sizeof(double) * (size_t)nblk_ * 5 * nbmax) ||
-        !alloc((void**)&cAB_, sizeof(double) * 2 * nbmax) || !alloc((void**)&scal_, sizeof(double) * SCAL_N * nbmax))
+        !alloc((void**)&cAB_, sizeof(double) * 2 * nbmax) || !alloc((void**)&scal_, sizeof(double) * SCAL_N * nbmax) ||
+        !alloc((void**)&rn_, sizeof(double) * 3 * nbmax))
         return "device allocation failed (propagator buffers)";
     for (int i = 0; i < 3; ++i) {
         e = upload_twiddles<T>(&tw_[i], m[i]);
@@ -628,6 +630,10 @@ hipError_t PropEngine<T>::run(const double* dW, long long wstride, double* dphi,
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_phis, dim3(nb), dim3(RED_NT), 0, st, part_, nblk_, M_, d_phis);
+    if (std::is_same<T, float>::value && cc.phihA == 0.0) {
+        hipLaunchKernelGGL(k_renorm_factors, dim3(nb), dim3(64), 0, st, cc, scal_, d_phis, rn_);
+        hipLaunchKernelGGL(k_renorm_apply, rg, dim3(RED_NT), 0, st, M_, rn_, dphi, phistride, d_slot2rep);
+    }
     return hipGetLastError();
 }
 

@@ -160,6 +160,7 @@ private:
     double* gpart_ = nullptr;
     double* cAB_ = nullptr;
     double* scal_ = nullptr;
+    double* rn_ = nullptr;  // FP32 normalisation factors [nbmax][3] (k_renorm_factors)
     size_t bytes_ = 0;
     bool timing_ = false;
     hipEvent_t ev_[2] = {nullptr, nullptr};

@@ -656,6 +656,62 @@ __global__ void __launch_bounds__(RED_NT) k_phis(const double* __restrict__ part
     }
 }
 
+// Weight total 2 + sum_{s=1}^{n-1} w_s of one Simpson sum (density.h:405-422: end terms 1,
+// interior 4, 2, 4, ... from the first interior point, whatever the parity).
+__host__ __device__ inline double simpson_total(int n)
+{
+    double t = 2.0;
+    for (int s = 1; s < n; ++s) t += (s & 1) ? 4.0 : 2.0;
+    return t;
+}
+
+// FP32 normalisation compensation. Every Richardson step is a symmetric operator, so
+// <q1'[s] q2'[L-s]> equals QC' (L = N) or Q2' (L = Nh) for every s, and each Simpson sum of
+// k_density has an exact mean: <rCA> = sC QC' W(n1), <rCB> = sC QC' W(n2), <rhB> = sH Q2' W(Nh).
+// With FP32 storage the invariant drifts with s by ~1e-7 (the plane spectra round the large
+// low-k coefficients on every store), which shifts the density normalisation and the free
+// energy by ~1e-6. fac[slot] = (exact mean / computed mean) of (rCA, rCB, rhB); in exact
+// arithmetic all three are 1. phis (the computed means) are set to the exact ones.
+// Used without the A homopolymer (phi rows then hold rCA, rhB and rCB separately).
+__global__ void k_renorm_factors(ChainConst cc, const double* __restrict__ scal, double* __restrict__ phis,
+                                 double* __restrict__ fac)
+{
+    const int slot = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    const double* o = scal + (size_t)slot * SCAL_N;
+    double* p = phis + (size_t)slot * 5;
+    const double sC = o[1], sH = o[2], QCp = o[6], Q2p = o[7];
+    const double tA = sC * QCp * simpson_total(cc.n1), tB = sC * QCp * simpson_total(cc.n2);
+    const double tH = sH * Q2p * simpson_total(cc.Nh);
+    const double mA = p[1] - p[3], mB = p[0] - mA, mH = p[4];  // <rCA>, <rCB>, <rhB>
+    double* f = fac + (size_t)slot * 3;
+    f[0] = mA != 0.0 ? tA / mA : 1.0;
+    f[1] = mB != 0.0 ? tB / mB : 1.0;
+    f[2] = mH != 0.0 ? tH / mH : 1.0;
+    p[0] = tA + tB;
+    p[1] = tA;
+    p[2] = tH + tB;
+    p[3] = 0.0;
+    p[4] = tH;
+}
+
+// phi rows (phiA = rCA, phiB = rhB + rCB, phih = rhB, phicB = rCB) scaled by fac
+__global__ void __launch_bounds__(RED_NT) k_renorm_apply(long long M, const double* __restrict__ fac,
+                                                         double* __restrict__ phi, long long phistride,
+                                                         const int* __restrict__ slot2rep)
+{
+    const int slot = blockIdx.y;
+    const double fA = fac[3 * slot], fB = fac[3 * slot + 1], fH = fac[3 * slot + 2];
+    double* ph = phi + (size_t)slot2rep[slot] * (size_t)phistride;
+    for (long long r = (long long)blockIdx.x * RED_NT + threadIdx.x; r < M; r += (long long)gridDim.x * RED_NT) {
+        const double h = ph[r + 2 * M] * fH, cB = ph[r + 3 * M] * fB;
+        ph[r] *= fA;
+        ph[r + M] = h + cB;
+        ph[r + 2 * M] = h;
+        ph[r + 3 * M] = cB;
+    }
+}
+
 // Diagnostics: part [s][nblk] = sum_r q1'[s](r) q2'[N-s](r) for one slot (A4 invariant).
 template <typename T>
 __global__ void __launch_bounds__(RED_NT) k_qq(const cpx<T>* __restrict__ qs, long long M, int N, int slot,

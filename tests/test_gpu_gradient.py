@@ -233,3 +233,67 @@ def test_gradient_at_64cube_N100(D0):
     assert _rel(g64, fd) <= 1e-6
     assert _rel(g64c, g64) <= 1e-12
     assert _rel(g32, g64) <= 1e-4
+
+
+def _string_64(R, D0):
+    """lamella -> cylinder string at 64^3 (the adjoint workload's fields), R replicas"""
+    m = (64, 64, 64)
+    M = int(np.prod(m))
+    X, Y = np.meshgrid(2 * np.pi * np.arange(m[0]) / m[0], 2 * np.pi * np.arange(m[1]) / m[1], indexing="ij")
+    lam = np.cos(X)
+    cyl = (np.cos(X) + np.cos(X / 2 + np.sqrt(3) * Y / 2) + np.cos(X / 2 - np.sqrt(3) * Y / 2)) / 3
+    W = np.zeros((R, 2, M))
+    for i in range(R):
+        t = i / (R - 1)
+        W[i, 0] = np.repeat((((1 - t) * lam + t * cyl) * 1.5)[:, :, None], m[2], axis=2).reshape(-1)
+    return m, W
+
+
+def test_string_gradient_8_replicas_64cube_N100():
+    """The 8-replica string of the adjoint workload (64^3, N = 100, chiN = 20, f = 0.5,
+    lamella -> cylinder, D = 6.0) in one batched call: FP64 d lnQ/dD of replicas 0, 3 and 7
+    against the oracle (1e-10), every replica of the batch against that replica alone
+    (1e-12: batching changes nothing), and FP32 against FP64 (1e-4)."""
+    R, D0, N, f, alpha, ens, phidb = 8, 6.0, 100, 0.5, 0.1, 1, 0.8
+    m, W = _string_64(R, D0)
+    D = (D0, D0, D0)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        with Context(m, D, N, f, alpha, ens, phidb, chi=20.0, nrep=R, precision=prec, fix=(1, 1)) as ctx:
+            res[prec] = ctx.dlnq_dbox(W)[1]
+            if prec == "fp64":
+                single = ctx.dlnq_dbox(W[3:4])[1][0]
+    orc.set_threads(16)
+    g = orc.Grid(m, D)
+    for i in (0, 3, 7):
+        _, dl = g.dlnq_dbox(W[i], N, alpha, phidb, f, ens)
+        _say(f"string replica {i}: gpu {res['fp64'][i]}, oracle {dl}")
+        assert _rel(res["fp64"][i], dl) <= 1e-10
+    assert _rel(single, res["fp64"][3]) <= 1e-12
+    worst = max(_rel(res["fp32"][i], res["fp64"][i]) for i in range(R))
+    _say(f"8-replica string: FP32 against FP64 worst rel {worst:.1e}")
+    assert worst <= 1e-4
+
+
+def test_relax_box_64cube_N100_to_zero_stress():
+    """A stress-free relaxation at the adjoint workload's size: a lamella along x in a 64^3
+    box (chiN = 20, f = 0.5, phidb = 0.8, N = 100, canonical, FP64), box length x relaxed
+    to |dF/dD_x| < 1e-6, the fields converged at every box by Anderson mixing. The oracle's
+    stress at the final fields and box confirms the value."""
+    m, N, f, alpha, ens, phidb, chi = (64, 64, 64), 100, 0.5, 0.1, 1, 0.8, 20.0
+    D0 = (4.2, 4.0, 4.0)
+    M = int(np.prod(m))
+    x = (np.arange(m[0]) * D0[0] / m[0])[:, None, None] * np.ones(m)
+    W0 = np.zeros((1, 2, M))
+    W0[0, 0] = (2.0 * np.cos(2 * np.pi * x / D0[0])).reshape(-1)
+    with Context(m, D0, N, f, alpha, ens, phidb, chi, dostring=0, anderson=20) as ctx:
+        ctx.set_fields(W0)
+        D, gst, nbox = ctx.relax_box(axes=(1, 0, 0), stress_tol=1e-6, err_tol=1e-9, max_field_iter=4000,
+                                     max_boxes=30)
+        W = ctx.get_fields()[0]
+    _say(f"64^3 lamella relaxed: D = {D[0]!r} after {nbox} boxes, dF/dD = {gst}")
+    assert abs(gst[0]) < 1e-6
+    orc.set_threads(16)
+    _, dl = orc.Grid(m, D).dlnq_dbox(W.reshape(2, -1), N, alpha, phidb, f, ens)
+    assert abs(-dl[0] - gst[0]) <= 1e-9
+    assert abs(dl[1]) < 1e-9 and abs(dl[2]) < 1e-9   # uniform along y and z: no stress there
