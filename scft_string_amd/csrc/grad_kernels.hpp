@@ -208,14 +208,18 @@ __global__ void __launch_bounds__(RED_NT) k_gacc(const cpx<T>* __restrict__ K0, 
     const int slot = blockIdx.y;
     const size_t so = (size_t)slot * (size_t)M;
     double t[3] = {0, 0, 0};
-    for (long long r = (long long)blockIdx.x * RED_NT + threadIdx.x; r < M; r += (long long)gridDim.x * RED_NT) {
-        const int kz = (int)(r % m2);
-        const long long rq = r / m2;
-        const int ky = (int)(rq % m1), kx = (int)(rq / m1);
+    // one slot's field has M < 2^31 points: 32-bit index arithmetic (a 64-bit division per
+    // point cost more than the loads)
+    const int Mi = (int)M;
+    for (int r = blockIdx.x * RED_NT + threadIdx.x; r < Mi; r += gridDim.x * RED_NT) {
+        const int rq = r / m2;
+        const int kz = r - rq * m2;
+        const int kx = rq / m1;
+        const int ky = rq - kx * m1;
         const int nx = kx <= m0 / 2 ? kx : kx - m0, ny = ky <= m1 / 2 ? ky : ky - m1,
                   nz = kz <= m2 / 2 ? kz : kz - m2;
         const int mx = kx == 0 ? 0 : m0 - kx, my = ky == 0 ? 0 : m1 - ky, mz = kz == 0 ? 0 : m2 - kz;
-        const long long rm = ((long long)mx * m1 + my) * m2 + mz;
+        const int rm = (mx * m1 + my) * m2 + mz;
         const double k2x = (gc.kD[0] * nx) * (gc.kD[0] * nx), k2y = (gc.kD[1] * ny) * (gc.kD[1] * ny),
                      k2z = (gc.kD[2] * nz) * (gc.kD[2] * nz);
         const double kh = exp(-(k2x + k2y + k2z) * gc.ds / 12.0);

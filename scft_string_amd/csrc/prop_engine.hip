@@ -158,16 +158,48 @@ struct KS {
             hipLaunchKernelGGL((k_plane_end2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, c,
                                (int)PL_RESUME);
     }
+    static hipError_t gsetup_two()
+    {
+        if constexpr (two_ok) {
+            hipError_t e = hipFuncSetAttribute((const void*)k_gplane_init2<T, N1, N2>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2_plane);
+            if (e != hipSuccess) return e;
+            return hipFuncSetAttribute((const void*)k_gcol2<T, N0, N1, N2, TC>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2_col);
+        }
+        return hipErrorNotSupported;
+    }
+    // the four passes of one gradient term over a.nb slots (the fifth, k_gacc, is the
+    // caller's): Ch, Cf <- F2(E1/4 Z), F2(E1/2 Z); x: K0, Kf <- spectra, Ch <- K12 sandwich;
+    // Ch <- F2(E1/2 F2^-1 Ch) (plane_mid with both parts on species spec); x: K1 <- spectrum
+    static void gterm_two(const PropArgs<T>& a, int ib, int pbk, int ia, int pak, int spec, cpx<T>* K0,
+                          cpx<T>* K1, cpx<T>* Kf, hipStream_t st)
+    {
+        if constexpr (two_ok) {
+            hipLaunchKernelGGL((k_gplane_init2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, ib, pbk,
+                               ia, pak, spec);
+            hipLaunchKernelGGL((k_gcol2<T, N0, N1, N2, TC>), dim3(ntile, 2 * a.nb), dim3(CG::NT), lds2_col, st, a,
+                               a.nb, (int)GC_SPLIT, K0, Kf);
+            PropArgs<T> b = a;  // k_plane_mid2 takes the real part's species from s < n1, the
+            b.n1 = spec == 0 ? 1 : 0;  // imaginary part's from s < n2: at s = 0 both get spec
+            b.n2 = spec == 0 ? 0 : 1;
+            hipLaunchKernelGGL((k_plane_mid2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, b, 0);
+            hipLaunchKernelGGL((k_gcol2<T, N0, N1, N2, TC>), dim3(ntile, a.nb), dim3(CG::NT), lds2_col, st, a, a.nb,
+                               (int)GC_FWD, K1, (cpx<T>*)nullptr);
+        }
+    }
     static KernelSet<T> make()
     {
         KernelSet<T> k{N0,      N1,      N2,      &setup_legacy, &begin_legacy,  &step_legacy,
                        nullptr, nullptr, nullptr, "stockham",    &resume_legacy, nullptr,
-                       lds_plane > 80 * 1024, TC, alt_ok ? TC_ALT : 0, persist_ok};
+                       lds_plane > 80 * 1024, TC, alt_ok ? TC_ALT : 0, persist_ok, nullptr, nullptr};
         if (two_ok) {
             k.setup2 = &setup_two;
             k.begin2 = &begin_two;
             k.step2 = &step_two;
             k.resume2 = &resume_two;
+            k.gsetup2 = &gsetup_two;
+            k.gterm2 = &gterm_two;
         }
         if (!legacy_ok) {
             k.setup = &setup_two;
@@ -722,8 +754,12 @@ template <typename T>
 hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out, hipStream_t st)
 {
     if (nb <= 0) return hipSuccess;
-    if (!pencil_supported(m_[0], m_[1], m_[2])) return hipErrorNotSupported;
+    // the two-stage gradient passes where the props run on the two-stage kernels; the line
+    // pipeline otherwise (any 2-3-5 grid)
+    const bool two = ks_ && ks_->gterm2 && step_ == ks_->step2;
+    if (!two && !pencil_supported(m_[0], m_[1], m_[2])) return hipErrorNotSupported;
     hipError_t e;
+    if (two && (e = ks_->gsetup2()) != hipSuccess) return e;
     if (!gpart_) {
         const size_t fld = sizeof(cpx<T>) * (size_t)M_ * nbmax_;
         const size_t pb = sizeof(double) * (size_t)nbmax_ * 3 * nblk_ * 3;
@@ -763,6 +799,12 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
         double* pc = gpart_ + (size_t)chain * np;
         const int pbk = chain == 1 ? 1 : 0, pak = chain == 2 ? 0 : 1;
         const int spec = chain == 1 ? (s < cc.n2 ? 1 : 0) : (s < cc.n1 ? 0 : 1);
+        if (two) {
+            ks_->gterm2(a, ib, pbk, ia, pak, spec, K0_, K1_, Kf_, st);
+            hipLaunchKernelGGL((k_gacc<T>), dim3(nblk_, nb), dim3(RED_NT), 0, st, K0_, K1_, Kf_, M_, m_[0], m_[1],
+                               m_[2], gc, pc);
+            return;
+        }
         hipLaunchKernelGGL((k_gz<T>), gzg, dim3(PC_NT), lds, st, a, (int)GZ_INIT, ib, pbk, ia, pak, spec, pg.nlz);
         pencil_py<T, false>(a, nb, 2 * nb, p, lds, st);
         hipLaunchKernelGGL((k_gx<T>), dim3(m_[1] * ntx, 2 * nb), dim3(PC_NT), lds, st, a, (int)GX_SPLIT, nb, pg.tx,

@@ -30,6 +30,12 @@ struct KernelSet {
     int tc, tc_alt;
     // the two-stage set has the persistent plane_mid (k_plane_mid2p)
     bool persist_mid;
+    // one box-gradient term on the two-stage kernels (k_gplane_init2, k_gcol2, k_plane_mid2):
+    // b, a from q-store entries ib (part pbk), ia (part pak), species spec; spectra to K0, K1,
+    // Kf (nullptr when the grid has no two-stage set)
+    hipError_t (*gsetup2)();
+    void (*gterm2)(const PropArgs<T>&, int ib, int pbk, int ia, int pak, int spec, cpx<T>* K0, cpx<T>* K1,
+                   cpx<T>* Kf, hipStream_t);
 };
 
 // kernel set for an (m0, m1, m2) grid, or nullptr when no instantiation exists

@@ -451,6 +451,50 @@ k_plane_end2(PropArgs<T> a, int s, int mode)
     P::fwd(q, buf, twz, twy, a.Cf + fo, tid);
 }
 
+// First pass of one box-gradient term (grad_kernels.hpp) on the two-stage FFT: b = part pbk
+// of q-store entry ib and a = part pak of entry ia (0 real, 1 imaginary), both taking the
+// step operator of species spec (0 A, 1 B), packed as Z = b + i a;
+// Ch <- F2(E1/4 Z), Cf <- F2(E1/2 Z).
+template <typename T, int NY, int NZ>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT)))
+k_gplane_init2(PropArgs<T> a, int ib, int pbk, int ia, int pak, int spec)
+{
+    using P = P2<T, NY, NZ>;
+    constexpr int NT = P::NT, IZ1 = P::IZ1, A2 = P::A2, NE = NY * NZ;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* twz = buf + P::LDSN;
+    cpx<T>* twy = twz + NZ;
+    const int tid = threadIdx.x;
+    const int x = blockIdx.x, slot = blockIdx.y;
+    const size_t fo = (size_t)slot * (size_t)a.M + (size_t)x * NE;
+    p2_load_tw<T, NY, NZ>(twz, twy, a, tid, NT);
+    const cpx<T>* qb = a.qs + ((size_t)slot * (size_t)a.qpitch + (size_t)ib) * (size_t)a.M + (size_t)x * NE;
+    const cpx<T>* qa = a.qs + ((size_t)slot * (size_t)a.qpitch + (size_t)ia) * (size_t)a.M + (size_t)x * NE;
+    const T* ef = spec == 0 ? a.e4A : a.e4B;
+    cpx<T> v[IZ1][A2], h[IZ1][A2];
+#pragma unroll
+    for (int it = 0; it < IZ1; ++it) {
+        const int w = tid + it * NT;
+#pragma unroll
+        for (int a2 = 0; a2 < A2; ++a2) {
+            if (P::z1_valid(w)) {
+                const int e = P::z1_elem(w, a2);
+                const cpx<T> vb = qb[e], va = qa[e];
+                const T zb = pbk ? vb.y : vb.x, za = pak ? va.y : va.x, ev = ef[fo + e];
+                v[it][a2] = {Boltz<T>::mul4(zb, ev), Boltz<T>::mul4(za, ev)};
+                h[it][a2] = {Boltz<T>::mul2(zb, ev), Boltz<T>::mul2(za, ev)};
+            } else {
+                v[it][a2] = h[it][a2] = {T(0), T(0)};
+            }
+        }
+    }
+    __syncthreads();  // twiddles
+    P::fwd(v, buf, twz, twy, a.Ch + fo, tid);
+    __syncthreads();
+    P::fwd(h, buf, twz, twy, a.Cf + fo, tid);
+}
+
 // plane_mid on the two-stage FFT: Ch <- F2( E1/2 * F2^-1(Ch) )
 template <typename T, int NY, int NZ>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT))) k_plane_mid2(PropArgs<T> a, int s)
@@ -647,6 +691,126 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<N0, TC>::NT
             });
         }
     }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < I2; ++it) {
+        const int w = tid + it * NT;
+        if ((TC * X2) % NT == 0 || w < TC * X2) {
+            const int c = w % TC, k2 = w / TC;
+            sfor<X1>([&](auto nc) {
+                constexpr int n1 = decltype(nc)::value;
+                buf[c * CX + n1 * PX + k2] = u[it][n1];
+            });
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < I1; ++it) {
+        const int w = tid + it * NT;
+        if ((TC * X1) % NT == 0 || w < TC * X1) {
+            const int c = w % TC, n1 = w / TC, col = c0 + c;
+            sfor<X2>([&](auto kc) {
+                constexpr int k2 = decltype(kc)::value;
+                v[it][k2] = buf[c * CX + n1 * PX + k2];
+            });
+            rdft<T, X2, true>(v[it]);
+            if (NC % TC == 0 || col < NC)
+                sfor<X2>([&](auto nc) {
+                    constexpr int n2 = decltype(nc)::value;
+                    fld[(size_t)(n1 + X1 * n2) * NC + col] = v[it][n2];
+                });
+        }
+    }
+}
+
+// Column pass of a box-gradient term on the two-stage FFT. Field f < nh is Ch of slot f,
+// f >= nh is Cf of slot f - nh. Every field: forward FFT along x, the spectrum stored to
+// K0 (f < nh) or Kf (f >= nh) in natural (kx, ky, kz) order. GC_SPLIT, f < nh: then times
+// the half-step kinetic factor / M and inverse along x, back to Ch (as k_column2).
+enum GColMode { GC_SPLIT = 0, GC_FWD = 1 };
+
+template <typename T, int N0, int NY, int NZ, int TC>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<N0, TC>::NT)))
+k_gcol2(PropArgs<T> a, int nh, int mode, cpx<T>* __restrict__ K0, cpx<T>* __restrict__ Kf)
+{
+    using G = C2Geom<N0, TC>;
+    constexpr int NT = G::NT, X1 = G::X1, X2 = G::X2, I1 = G::I1, I2 = G::I2, CX = G::CX, PX = G::PX;
+    constexpr int NC = NY * NZ;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* tw0 = buf + G::LDSN;
+    T* gx = reinterpret_cast<T*>(tw0 + N0);
+    const int tid = threadIdx.x;
+    const int tile = blockIdx.x, f = blockIdx.y;
+    const bool isf = f >= nh;
+    const size_t so = (size_t)(isf ? f - nh : f) * (size_t)a.M;
+    cpx<T>* fld = (isf ? a.Cf : a.Ch) + so;
+    cpx<T>* ko = (isf ? Kf : K0) + so;
+    const int c0 = tile * TC;
+    for (int i = tid; i < N0; i += NT) {
+        tw0[i] = a.tw0[i];
+        gx[i] = a.gx[i];
+    }
+    cpx<T> v[I1][X2];
+#pragma unroll
+    for (int it = 0; it < I1; ++it) {
+        const int w = tid + it * NT;
+        if ((TC * X1) % NT == 0 || w < TC * X1) {
+            const int c = w % TC, n1 = w / TC, col = c0 + c;
+            const bool in = NC % TC == 0 || col < NC;
+            sfor<X2>([&](auto nc) {
+                constexpr int n2 = decltype(nc)::value;
+                v[it][n2] = in ? fld[(size_t)(n1 + X1 * n2) * NC + col] : cpx<T>{T(0), T(0)};
+            });
+        }
+    }
+    __syncthreads();  // twiddle / gx tables
+#pragma unroll
+    for (int it = 0; it < I1; ++it) {
+        const int w = tid + it * NT;
+        if ((TC * X1) % NT == 0 || w < TC * X1) {
+            const int c = w % TC, n1 = w / TC;
+            rdft<T, X2, false>(v[it]);
+            sfor<X2>([&](auto kc) {
+                constexpr int k2 = decltype(kc)::value;
+                buf[c * CX + n1 * PX + k2] = k2 == 0 ? v[it][0] : cmul(v[it][k2], tw0[(n1 * k2) % N0]);
+            });
+        }
+    }
+    __syncthreads();
+    const bool back = mode == GC_SPLIT && !isf;  // uniform over the workgroup
+    cpx<T> u[I2][X1];
+#pragma unroll
+    for (int it = 0; it < I2; ++it) {
+        const int w = tid + it * NT;
+        if ((TC * X2) % NT == 0 || w < TC * X2) {
+            const int c = w % TC, k2 = w / TC, col = c0 + c;
+            sfor<X1>([&](auto nc) {
+                constexpr int n1 = decltype(nc)::value;
+                u[it][n1] = buf[c * CX + n1 * PX + k2];
+            });
+            rdft<T, X1, false>(u[it]);
+            if (NC % TC == 0 || col < NC)
+                sfor<X1>([&](auto kc) {
+                    constexpr int k1 = decltype(kc)::value;
+                    ko[(size_t)(k2 + X2 * k1) * NC + col] = u[it][k1];
+                });
+            if (back) {
+                const int y = (col < NC ? col : 0) / NZ, z = (col < NC ? col : 0) % NZ;
+                const T gyz = Kinetic<T>::col(a.gy[y], a.gz[z]);
+                sfor<X1>([&](auto kc) {
+                    constexpr int k1 = decltype(kc)::value;
+                    Kinetic<T>::apply(u[it][k1], gx[k2 + X2 * k1], gyz, false, a.invM);
+                });
+                rdft<T, X1, true>(u[it]);
+                sfor<X1>([&](auto nc) {
+                    constexpr int n1 = decltype(nc)::value;
+                    if (n1 != 0) u[it][n1] = cmulc(u[it][n1], tw0[(n1 * k2) % N0]);
+                });
+            }
+        }
+    }
+    if (!back) return;
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < I2; ++it) {
