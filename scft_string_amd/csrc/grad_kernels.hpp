@@ -204,7 +204,6 @@ __global__ void __launch_bounds__(RED_NT) k_gacc(const cpx<T>* __restrict__ K0, 
                                                   const cpx<T>* __restrict__ Kf, long long M, int m0, int m1,
                                                   int m2, GradConst gc, double* __restrict__ part)
 {
-    __shared__ double sh[RED_NT];
     const int slot = blockIdx.y;
     const size_t so = (size_t)slot * (size_t)M;
     double t[3] = {0, 0, 0};
@@ -228,20 +227,26 @@ __global__ void __launch_bounds__(RED_NT) k_gacc(const cpx<T>* __restrict__ K0, 
         const double bf = (double)fm.x * (double)fp.y + (double)fm.y * (double)fp.x;
         // 4/3 * ds/(6 D_i) * kh * bh - 1/3 * ds/(3 D_i) * kh^2 * bf / 2, times k_i^2
         const double c = gc.ds * ((2.0 / 9.0) * kh * bh - (1.0 / 18.0) * kh * kh * bf);
-        t[0] += c * k2x / gc.D[0];
-        t[1] += c * k2y / gc.D[1];
-        t[2] += c * k2z / gc.D[2];
+        t[0] += c * k2x;
+        t[1] += c * k2y;
+        t[2] += c * k2z;
     }
+    // the 1/D_i of every term, once per thread; then each wave sums its 64 lanes by
+    // shuffles and one barrier hands the RED_NT / 64 wave sums to thread 0
+    __shared__ double wsum[3][RED_NT / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        sh[threadIdx.x] = t[i];
-        __syncthreads();
-        for (int o = RED_NT / 2; o > 0; o >>= 1) {
-            if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) part[((size_t)slot * gridDim.x + blockIdx.x) * 3 + i] += sh[0];
-        __syncthreads();
+        double v = t[i] / gc.D[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) wsum[i][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double v = 0;
+        for (int w = 0; w < RED_NT / 64; ++w) v += wsum[threadIdx.x][w];
+        part[((size_t)slot * gridDim.x + blockIdx.x) * 3 + threadIdx.x] += v;
     }
 }
 

@@ -776,6 +776,11 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
             if ((e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds)) != hipSuccess) return e;
     }
     const size_t np = (size_t)nb * nblk_ * 3;  // partials per chain
+    // k_gacc workgroups per slot: about 1024 over the launch (4 per CU), so that many
+    // replicas take several points per thread and the block sums stay few (one point per
+    // thread over 8 x 1024 workgroups cost 23 us per launch at 8 x 64^3, 128 workgroups
+    // per slot 17 us; one replica on 128 workgroups left half the CUs idle)
+    const int gblk = std::max(1, std::min(nblk_, std::max(128, 1024 / std::max(nb, 1))));
     if ((e = hipMemsetAsync(gpart_, 0, sizeof(double) * 3 * np, st)) != hipSuccess) return e;
 
     const PropArgs<T> a = args(nb, cc);
@@ -801,7 +806,7 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
         const int spec = chain == 1 ? (s < cc.n2 ? 1 : 0) : (s < cc.n1 ? 0 : 1);
         if (two) {
             ks_->gterm2(a, ib, pbk, ia, pak, spec, K0_, K1_, Kf_, st);
-            hipLaunchKernelGGL((k_gacc<T>), dim3(nblk_, nb), dim3(RED_NT), 0, st, K0_, K1_, Kf_, M_, m_[0], m_[1],
+            hipLaunchKernelGGL((k_gacc<T>), dim3(gblk, nb), dim3(RED_NT), 0, st, K0_, K1_, Kf_, M_, m_[0], m_[1],
                                m_[2], gc, pc);
             return;
         }
@@ -814,7 +819,7 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
         pencil_py<T, false>(a, nb, nb, p, lds, st);
         hipLaunchKernelGGL((k_gx<T>), dim3(m_[1] * ntx, nb), dim3(PC_NT), lds, st, a, (int)GX_FWD, nb, pg.tx, K1_,
                            (cpx<T>*)nullptr);
-        hipLaunchKernelGGL((k_gacc<T>), dim3(nblk_, nb), dim3(RED_NT), 0, st, K0_, K1_, Kf_, M_, m_[0], m_[1],
+        hipLaunchKernelGGL((k_gacc<T>), dim3(gblk, nb), dim3(RED_NT), 0, st, K0_, K1_, Kf_, M_, m_[0], m_[1],
                            m_[2], gc, pc);
     };
     const bool hA = cc.phihA != 0.0;  // chain 2 (the A homopolymer's Q1) only when it is present
@@ -854,7 +859,7 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
         for (int chain = 0; chain < 3; ++chain)
             for (int i = 0; i < 3; ++i) {
                 double v = 0;
-                for (int b = 0; b < nblk_; ++b) v += h[(size_t)chain * np + ((size_t)slot * nblk_ + b) * 3 + i];
+                for (int b = 0; b < gblk; ++b) v += h[(size_t)chain * np + ((size_t)slot * gblk + b) * 3 + i];
                 out[(size_t)slot * 9 + chain * 3 + i] = v * scale;
             }
     return hipSuccess;
