@@ -7,10 +7,40 @@
 // registers. Used for grids whose three axes all split as N1 x N2 with 1 < N1 <= N2 <= 16
 // (24 ... 128); smaller axes use the LDS Stockham kernels.
 #pragma once
+#include <type_traits>
+
 #include "fft2.hpp"
 #include "prop_types.hpp"
 
 namespace scft {
+
+// Sum of v over the 64 lanes of a wave in FP64, the same value in every lane; a fixed
+// butterfly order, so the result does not depend on timing.
+__device__ __forceinline__ double wave_sum64(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// The L outputs of an inverse DFT sum to exactly L times its DC input. In FP32 the partial
+// sums that carry the DC are rounded once and shared by all outputs, so the outputs' sum,
+// and with it the mean of the inverse transform, is off by ~1 ulp of the DC per transform,
+// coherently. Folding the FP64 residual of the sum (against L (tx, ty), the DC input or its
+// FP64 value where the kernel has one) into u[0] leaves half an ulp of one output. Used on
+// the items that carry the 3-D DC (the mean of q).
+template <typename T, int L>
+__device__ __forceinline__ void inv_sum_fix(cpx<T>* u, double tx, double ty)
+{
+    double sx = 0.0, sy = 0.0;
+#pragma unroll
+    for (int b = 0; b < L; ++b) {
+        sx += (double)u[b].x;
+        sy += (double)u[b].y;
+    }
+    u[0].x = (T)((double)u[0].x + ((double)L * tx - sx));
+    u[0].y = (T)((double)u[0].y + ((double)L * ty - sy));
+}
 
 constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
@@ -652,6 +682,23 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<N0, TC>::NT
             });
         }
     }
+    // FP32: the x transform's DC coefficient of column (0, 0), i.e. the 3-D DC, is the FP64
+    // sum of the column's FP32 values, rounded once. Wave 0 of tile 0 loads the column a
+    // second time, one value per lane, with the loads above (so that its latency is theirs),
+    // and adds it up after stage 1, when few registers are live.
+    constexpr bool DC64 = std::is_same<T, float>::value;
+    constexpr int NPL = (N0 + 63) / 64;
+    __shared__ double dcx[2];
+    cpx<T> dcv[NPL];
+    if constexpr (DC64) {
+        if (tile == 0 && tid < 64) {
+#pragma unroll
+            for (int j = 0; j < NPL; ++j) {
+                const int x = tid + 64 * j;
+                dcv[j] = x < N0 ? fld[(size_t)x * NC] : cpx<T>{T(0), T(0)};
+            }
+        }
+    }
     __syncthreads();  // twiddle / gx tables
 #pragma unroll
     for (int it = 0; it < I1; ++it) {
@@ -663,6 +710,22 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<N0, TC>::NT
                 constexpr int k2 = decltype(kc)::value;
                 buf[c * CX + n1 * PX + k2] = k2 == 0 ? v[it][0] : cmul(v[it][k2], tw0[(n1 * k2) % N0]);
             });
+        }
+    }
+    if constexpr (DC64) {
+        if (tile == 0 && tid < 64) {
+            double sx = 0.0, sy = 0.0;
+#pragma unroll
+            for (int j = 0; j < NPL; ++j) {
+                sx += (double)dcv[j].x;
+                sy += (double)dcv[j].y;
+            }
+            sx = wave_sum64(sx);
+            sy = wave_sum64(sy);
+            if (tid == 0) {
+                dcx[0] = sx;
+                dcx[1] = sy;
+            }
         }
     }
     __syncthreads();
@@ -677,6 +740,9 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<N0, TC>::NT
                 u[it][n1] = buf[c * CX + n1 * PX + k2];
             });
             rdft<T, X1, false>(u[it]);
+            if constexpr (DC64) {
+                if (tile == 0 && w == 0) u[it][0] = {(T)dcx[0], (T)dcx[1]};  // column 0, k2 = 0: kx = 0
+            }
             // kinetic factor at kx = k2 + X2 k1 (exp(-k^2 ds/12)/M, or its square * M)
             const int y = (col < NC ? col : 0) / NZ, z = (col < NC ? col : 0) % NZ;
             const T gyz = Kinetic<T>::col(a.gy[y], a.gz[z]);
@@ -685,6 +751,12 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<N0, TC>::NT
                 Kinetic<T>::apply(u[it][k1], gx[k2 + X2 * k1], gyz, full, a.invM);
             });
             rdft<T, X1, true>(u[it]);
+            if constexpr (DC64) {
+                // column 0, k2 = 0: the target is the FP64 3-D DC times the kinetic factor at
+                // k = 0 (1/M), not its FP32 rounding, which would walk the mean by ~0.3 ulp
+                // per column pass
+                if (tile == 0 && w == 0) inv_sum_fix<T, X1>(u[it], dcx[0] * (double)a.invM, dcx[1] * (double)a.invM);
+            }
             sfor<X1>([&](auto nc) {
                 constexpr int n1 = decltype(nc)::value;
                 if (n1 != 0) u[it][n1] = cmulc(u[it][n1], tw0[(n1 * k2) % N0]);
@@ -713,7 +785,11 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<N0, TC>::NT
                 constexpr int k2 = decltype(kc)::value;
                 v[it][k2] = buf[c * CX + n1 * PX + k2];
             });
+            const cpx<T> vin0 = v[it][0];
             rdft<T, X2, true>(v[it]);
+            if constexpr (DC64) {
+                if (tile == 0 && c == 0) inv_sum_fix<T, X2>(v[it], vin0.x, vin0.y);  // column 0
+            }
             if (NC % TC == 0 || col < NC)
                 sfor<X2>([&](auto nc) {
                     constexpr int n2 = decltype(nc)::value;
