@@ -9,7 +9,11 @@ interpolants plus N(0, 0.01^2) noise (seed 2024); W+ = 0. A step is one solve_fi
 (scft.cu:592-806): W+ relaxation (<= 19 props) and one props per active replica, DEV, errors,
 perpendicular projection, mixing, arc length, spline redistribution, with FreeE at k % 50 == 0.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-autotune]
+
+The propagator schedule (kernel family, column tile, plane workgroup size, replica-group
+streams) is the autotuner's choice for this string (scft_autotune, run before the timed
+region); --no-autotune keeps the defaults.
 
 Other modes, each printing one JSON line of its own:
 
@@ -22,6 +26,7 @@ Other modes, each printing one JSON line of its own:
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -118,11 +123,19 @@ def precision_report(W, Wshape_m, c, oracle_rows=(0,)):
             ctx.set_fields(W)
             fe[prec] = ctx.free_energy()["FEs"]  # FreeE: W+ relaxation + props (F in FP64)
         res[prec] = (phi, lnQ, free_energies(W, phi, lnQ, c["chi"]))
+    # the same FreeE with every props in FP32 (fe_fp64 = 0): Q, F and the sums in FP64
+    with Context(c["m"], c["D"], c["N"], c["f"], c["alpha"], c["ens"], c["phidb"], c["chi"], nrep=R,
+                 precision="fp32", fix=c["fix"], fe_fp64=0) as ctx:
+        ctx.set_fields(W)
+        fe["fp32_all"] = ctx.free_energy()["FEs"]
     # FreeE of the FP32 context (FP32 W+ relaxation, F from an FP64 props) vs the FP64 context
     dFE = np.abs(fe["fp32"] - fe["fp64"])
     rel = (fe["fp32"] - fe["fp32"][0]) - (fe["fp64"] - fe["fp64"][0])
     out["freeE_fp32_vs_fp64_max_abs_dF"] = float(dFE.max())
     out["freeE_fp32_vs_fp64_max_abs_dF_between_replicas"] = float(np.abs(rel).max())
+    rel32 = (fe["fp32_all"] - fe["fp32_all"][0]) - (fe["fp64"] - fe["fp64"][0])
+    out["freeE_all_fp32_vs_fp64_max_abs_dF"] = float(np.abs(fe["fp32_all"] - fe["fp64"]).max())
+    out["freeE_all_fp32_vs_fp64_max_abs_dF_between_replicas"] = float(np.abs(rel32).max())
     out["FEs_fp64"] = [float(v) for v in fe["fp64"]]
     # one FP32 props against one FP64 props on identical fields (first order in FP32 roundoff)
     dF = np.abs(res["fp32"][2] - res["fp64"][2])
@@ -212,9 +225,10 @@ def plain_fp64_rate(W, c, props_per_iter, reps=2):
                       "props per string iteration"}
 
 
-def pmc_probe():
+def pmc_probe(schedule=None):
     """Workload for rocprofv3 --pmc runs: a 512 MiB device copy of known size (the scale for
-    FETCH_SIZE / WRITE_SIZE) followed by one batched props of the benchmark string."""
+    FETCH_SIZE / WRITE_SIZE) followed by one batched props of the benchmark string, on the
+    given propagator schedule (None: the defaults)."""
     import torch
 
     from scft_string_amd import Context
@@ -228,7 +242,7 @@ def pmc_probe():
     c = CFG
     R = c["nrep"]
     ctx = Context(c["m"], c["D"], c["N"], c["f"], c["alpha"], c["ens"], c["phidb"], c["chi"], nrep=R,
-                  precision="fp32", fix=c["fix"], dostring=1)
+                  precision="fp32", fix=c["fix"], dostring=1, schedule=schedule)
     ctx.set_fields(initial_fields(c["m"], c["D"], R))
     ctx.time_props(R - sum(c["fix"]), reps=1)
     ctx.close()
@@ -236,10 +250,12 @@ def pmc_probe():
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_contour_step_14x64cube_N128_fp32.json")
-CONTOUR_KERNELS = ("k_column2<float, 64, 64, 64, 64>", "k_plane_mid2<float, 64, 64>", "k_plane_end2<float, 64, 64>")
+# the contour loop's kernels on any schedule: column / plane (two-stage and Stockham) and line
+# passes; the densities, reductions and factors are not part of a contour step
+CONTOUR_KERNEL = re.compile(r"scft::k_(column2?|plane_(?:mid|end)2?p?|p[xyz])<float")
 
 
-def measure_traffic(N, timeout=150):
+def measure_traffic(N, schedule=None, timeout=150):
     """Memory traffic of one contour step of the benchmark batch, measured now: two child
     runs of `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` over `bench.py --pmc-probe`
     (a 512 MiB copy, then one props of the 14 active replicas). Each counter is scaled by what
@@ -258,7 +274,7 @@ def measure_traffic(N, timeout=150):
         for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
             out = os.path.join(td, ctr)
             cmd = [exe, "--pmc", ctr, "--output-format", "csv", "-d", out, "-o", "pmc", "--", sys.executable,
-                   os.path.join(ROOT, "bench.py"), "--pmc-probe"]
+                   os.path.join(ROOT, "bench.py"), "--pmc-probe", "--schedule", json.dumps(schedule or {})]
             try:
                 r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), capture_output=True,
                                    text=True, timeout=timeout)
@@ -274,11 +290,13 @@ def measure_traffic(N, timeout=150):
                 name, v = row["Kernel_Name"], float(row["Counter_Value"])
                 if name.startswith("__amd_rocclr_copyBuffer") and int(row["Grid_Size"]) >= 1 << 16:
                     copy_kib = v
-                elif any(k in name for k in CONTOUR_KERNELS):
+                elif CONTOUR_KERNEL.search(name):
                     kern_kib += v
                     disp += 1
             if not copy_kib:
                 return None, f"no copy kernel in the {ctr} run"
+            if disp < 4 * N:  # at least the four launches of every contour step
+                return None, f"{disp} contour-kernel dispatches in the {ctr} run (expected at least 4 N = {4 * N})"
             scale = (512 << 20) / (copy_kib * 1024.0)
             tot[ctr] = (kern_kib * 1024.0 * scale / N, scale, disp)
     byte = tot["FETCH_SIZE"][0] + tot["WRITE_SIZE"][0]
@@ -480,6 +498,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--pmc-probe", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc traffic measurement")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="run the default propagator schedule instead of the autotuner's choice")
+    ap.add_argument("--schedule", default=None, help=argparse.SUPPRESS)  # --pmc-probe: JSON schedule
     ap.add_argument("--adjoint", action="store_true",
                     help="box-gradient timing: 64^3, N=100, FP32, D swept over 5.6-6.4, 1 and 8 replicas")
     ap.add_argument("--uneven", action="store_true",
@@ -495,7 +516,7 @@ def main():
                           "n_gpus": a.gpus}))
         sys.exit(2)
     if a.pmc_probe:
-        pmc_probe()
+        pmc_probe(json.loads(a.schedule) if a.schedule else None)
         return
     if a.adjoint:
         adjoint_report()
@@ -510,13 +531,23 @@ def main():
     import torch
 
     from scft_string_amd import Context
+    from scft_string_amd.capi import autotune
 
     torch.cuda.init()
     c = CFG
     R = c["nrep"]
     W0 = initial_fields(c["m"], c["D"], R)
+    sched, tuned = None, None
+    if not a.no_autotune:
+        # the propagator schedule: every candidate timed on this string, each in a context of
+        # its own, before the benchmark's context exists (outside the timed region)
+        t0 = time.perf_counter()
+        sched, report = autotune(c["m"], c["D"], c["N"], c["f"], c["alpha"], c["ens"], c["phidb"], c["chi"], nrep=R,
+                                 precision=a.precision, fix=c["fix"], dostring=1, W=W0, reps=2)
+        tuned = {"schedule": sched, "candidates": report, "seconds": round(time.perf_counter() - t0, 2)}
+        print(f"bench: autotune {report}", file=sys.stderr, flush=True)
     ctx = Context(c["m"], c["D"], c["N"], c["f"], c["alpha"], c["ens"], c["phidb"], c["chi"], nrep=R,
-                  precision=a.precision, fix=c["fix"], dostring=1)
+                  precision=a.precision, fix=c["fix"], dostring=1, schedule=sched)
     ctx.set_fields(W0)
     print(f"bench: {R} replicas {c['m']} N={c['N']} {a.precision}; warmup {a.warmup}", file=sys.stderr, flush=True)
 
@@ -546,12 +577,12 @@ def main():
     bstep = algorithmic_bytes_per_step(c["m"], p) * n_act
     achieved = bstep / (ms_step * 1e-3) / 1e9
     Wend = ctx.get_fields()
-    engine = ctx.engine_info.split(";")[0]
+    engine = ctx.engine_info.split(";")[0] + (" (autotuned)" if tuned else "")
     ctx.close()
 
     traffic, traffic_info = None, None
     if a.precision == "fp32" and not a.no_traffic:
-        traffic, traffic_info = measure_traffic(c["N"])
+        traffic, traffic_info = measure_traffic(c["N"], sched)
         if traffic is None and os.path.exists(TRAFFIC_PROFILE):   # fall back to the committed profile
             why = traffic_info
             traffic = json.load(open(TRAFFIC_PROFILE)).get("hbm_bytes_per_contour_step")
@@ -572,9 +603,11 @@ def main():
                                f"{n_act} replicas", "ms_per_contour_step": ms_step,
                      "algorithmic_bytes_per_contour_step": bstep, "traffic_info": traffic_info,
                      "time_base": "HIP events on the context stream around the contour loop of props calls "
-                                  "(two replica-group streams overlapping); the one-stream kernel-trace averages "
-                                  "are in profiles/kernel_trace_string_16x64cube_N128_fp32.md"},
+                                  "(the schedule's replica-group streams overlapping); the one-stream kernel-trace "
+                                  "averages are in profiles/kernel_trace_string_16x64cube_N128_fp32.md"},
     }
+    if tuned:
+        out["autotune"] = tuned
     if a.precision == "fp32" and not a.no_fp64:
         out["batched_fp64"] = batched_fp64_rate(W0, a.warmup, 2)
     if not a.no_plain:

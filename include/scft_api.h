@@ -19,6 +19,8 @@
 #ifndef SCFT_API_H
 #define SCFT_API_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -40,6 +42,25 @@ typedef enum scft_precision {
 
 /* Run parameters: the keys of the reference's input.dat (scft.cu:120-251) plus the
  * compile-time replica count strn (scft.cu:35), made a runtime value here. */
+/* Propagator schedule: which kernels run the contour loop and how they are tiled. The
+ * reference has no counterpart (cuFFT picks its own plans, density.h:41-75). A zero field
+ * keeps the default: the grid's measured choice, or the SCFT_* environment override. A
+ * non-zero field the grid cannot run makes scft_ctx_create return SCFT_ERR_INVALID. */
+typedef enum {
+    SCFT_SCHED_DEFAULT = 0,
+    SCFT_SCHED_TWO_STAGE = 1, /* plane/column kernels on register FFTs (grids compiled for them) */
+    SCFT_SCHED_STOCKHAM = 2,  /* plane/column kernels on LDS Stockham FFTs */
+    SCFT_SCHED_PENCIL = 3     /* one line pass per axis, any grid whose axes factor into 2, 3, 5 */
+} scft_sched_family;
+
+typedef struct scft_schedule {
+    int family;   /* scft_sched_family */
+    int coltile;  /* two-stage: (y, z) columns per workgroup of the x pass (32 or 64) */
+    int plane_nt; /* two-stage: threads per workgroup of the plane kernels (the grid's default
+                     or half of it, for 512- and 768-thread planes) */
+    int streams;  /* replica groups on their own streams, 1 .. 4 */
+} scft_schedule;
+
 typedef struct scft_params {
     int m[3];        /* grid points, x slowest */
     double D[3];     /* box lengths in R0 */
@@ -70,17 +91,16 @@ typedef struct scft_params {
     double anderson_start;   /* simple mixing (lambda 0.05) until errs < this (<= 0: 1e-2;
                                 started much earlier, Anderson can land on another saddle
                                 point, e.g. the disordered one) */
-    int autotune;            /* DISABLED: scft_ctx_create returns SCFT_ERR_UNSUPPORTED for 1.
-                                Autotuned contexts crashed on the host in their first props
-                                with the cause not found. (Meant: the first props times every
-                                kernel family x replica-group streams x graph replay on that
-                                batch and keeps the fastest.) 0 = the measured defaults. */
+    int autotune;            /* 1: scft_ctx_create first runs scft_autotune for these
+                                parameters (a lamellar trial field) and creates the context
+                                with the fastest schedule; sched is ignored. 0 = sched. */
     int ckpt;                /* 0: keep the propagators of every contour index (2(N+1) fields
                                 per replica, as density.h:293). K > 0: keep every K-th index
                                 (K ~ sqrt(N)) and recompute the rest in windows of K for the
                                 densities and the box gradient: N/K + 2K + 5 fields per
                                 replica, about twice the propagation work. Results agree with
                                 K = 0 to rounding. */
+    scft_schedule sched;     /* propagator schedule (all zero: the defaults) */
 } scft_params;
 
 typedef struct scft_ctx scft_ctx;
@@ -101,6 +121,21 @@ const char *scft_fft_path(const scft_ctx *ctx);
 /* The propagator configuration in words ("two-stage, 2 replica-group streams, direct
  * launches"), followed after an autotune by every candidate's contour-loop time in ms. */
 const char *scft_engine_info(scft_ctx *ctx);
+
+/* Autotuner (no counterpart in the reference, whose cuFFT plans are fixed per grid): times
+ * the contour loop of one props over p->nrep rows for every schedule the grid can run (kernel
+ * family; for the two-stage kernels, column tile x plane workgroup size; replica-group
+ * streams 1 .. min(3, nrep)), each in a context of its own that is created, timed and
+ * destroyed in turn, so no context is ever reconfigured. The fields are W [nrep][2M] when W
+ * is non-NULL, else a lamellar trial field. Each candidate: one untimed props, then the
+ * best of `reps` (>= 1) timed ones; candidates within 3% of the fastest are timed again in a
+ * fresh context (reps + 1 runs) and keep their best time. The rows timed are those a string
+ * iteration propagates (nrep less the fixed ends when dostring = 1 and justFE = 0). *best
+ * receives the fastest schedule; report (may be NULL) every candidate with its ms per
+ * contour step and the choice, NUL-terminated within report_len. p->sched and p->autotune
+ * are ignored. */
+scft_status scft_autotune(const scft_params *p, const double *W, int reps, scft_schedule *best,
+                          char *report, size_t report_len);
 
 /* Replaces density::props (density.h:344-459), batched over `nrep` independent rows:
  * W [nrep][2M] in; phi [nrep][4M], lnQ [nrep], phis [nrep][5] out (phi, phis may be NULL).

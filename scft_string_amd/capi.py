@@ -22,7 +22,7 @@ EXPORTED = [
     "scft_debug_qq_invariant", "scft_debug_time_props", "scft_fft_path", "scft_dlnq_dbox",
     "scft_box_stress", "scft_set_box", "scft_relax_box", "scft_iterate_concurrent", "scft_engine_info",
     "scft_backward", "scft_spline_setup", "scft_spline_set_x", "scft_spline_set_xref", "scft_spline_derivs",
-    "scft_spline_redist", "scft_spline_weights", "scft_spline_derivs_noalloc",
+    "scft_spline_redist", "scft_spline_weights", "scft_spline_derivs_noalloc", "scft_autotune",
 ]
 _STRING_RESULT = ("scft_last_error", "scft_fft_path", "scft_engine_info")
 
@@ -33,6 +33,33 @@ class ScftError(RuntimeError):
         self.code = code
 
 
+SCFT_SCHED_DEFAULT, SCFT_SCHED_TWO_STAGE, SCFT_SCHED_STOCKHAM, SCFT_SCHED_PENCIL = range(4)
+_FAMILY = {"default": SCFT_SCHED_DEFAULT, "two-stage": SCFT_SCHED_TWO_STAGE, "stockham": SCFT_SCHED_STOCKHAM,
+           "pencil": SCFT_SCHED_PENCIL}
+
+
+class Schedule(C.Structure):
+    """scft_schedule: kernel family, two-stage column tile and plane workgroup size,
+    replica-group streams (0 in a field: the default)."""
+    _fields_ = [("family", C.c_int), ("coltile", C.c_int), ("plane_nt", C.c_int), ("streams", C.c_int)]
+
+    def as_dict(self):
+        fam = {v: k for k, v in _FAMILY.items()}[self.family]
+        return {"family": fam, "coltile": self.coltile, "plane_nt": self.plane_nt, "streams": self.streams}
+
+
+def make_schedule(sched):
+    """Schedule from None, a Schedule, or a dict with any of family ("two-stage", "stockham",
+    "pencil"), coltile, plane_nt, streams."""
+    if sched is None:
+        return Schedule()
+    if isinstance(sched, Schedule):
+        return sched
+    fam = sched.get("family", "default")
+    return Schedule(family=_FAMILY[fam] if isinstance(fam, str) else int(fam), coltile=int(sched.get("coltile", 0)),
+                    plane_nt=int(sched.get("plane_nt", 0)), streams=int(sched.get("streams", 0)))
+
+
 class Params(C.Structure):
     _fields_ = [
         ("m", C.c_int * 3), ("D", C.c_double * 3), ("N", C.c_int), ("f", C.c_double),
@@ -40,7 +67,7 @@ class Params(C.Structure):
         ("nrep", C.c_int), ("precision", C.c_int), ("fix", C.c_int * 2), ("dostring", C.c_int),
         ("justFE", C.c_int), ("springsteps", C.c_int), ("device", C.c_int), ("fe_fp64", C.c_int),
         ("phihA", C.c_double), ("anderson", C.c_int), ("anderson_lambda", C.c_double),
-        ("anderson_start", C.c_double), ("autotune", C.c_int), ("ckpt", C.c_int),
+        ("anderson_start", C.c_double), ("autotune", C.c_int), ("ckpt", C.c_int), ("sched", Schedule),
     ]
 
 
@@ -83,6 +110,7 @@ def lib():
         L.scft_fft_path.restype = C.c_char_p
         L.scft_engine_info.argtypes = [vp]
         L.scft_engine_info.restype = C.c_char_p
+        L.scft_autotune.argtypes = [C.POINTER(Params), dp, C.c_int, C.POINTER(Schedule), C.c_char_p, C.c_size_t]
         L.scft_dlnq_dbox.argtypes = [vp, dp, C.c_int, dp, dp]
         L.scft_box_stress.argtypes = [vp, dp, dp]
         L.scft_backward.argtypes = [vp, dp, dp, dp]
@@ -130,12 +158,34 @@ def spline_weights(x, xe, erows, deriv):
     return G
 
 
+def autotune(m, D, N, f, alpha, ens, phidb, chi, nrep=1, precision="fp64", fix=(0, 0), dostring=1, W=None, reps=2,
+             ckpt=0, device=0):
+    """scft_autotune: times every schedule the grid can run, each in a context of its own, on
+    W ((nrep, 2, M); None: a lamellar trial field). Returns (schedule dict, report text)."""
+    prec = {"fp64": SCFT_FP64, "fp32": SCFT_FP32}[precision]
+    p = Params(m=(C.c_int * 3)(*[int(v) for v in m]), D=(C.c_double * 3)(*[float(v) for v in D]), N=int(N),
+               f=float(f), alpha=float(alpha), ens=int(ens), phidb=float(phidb), chi=float(chi), nrep=int(nrep),
+               precision=prec, fix=(C.c_int * 2)(*fix), dostring=int(dostring), device=int(device), fe_fp64=1,
+               ckpt=int(ckpt))
+    Wp = None
+    if W is not None:
+        W = np.ascontiguousarray(W, dtype=np.float64).reshape(int(nrep), -1)
+        Wp = _dp(W)
+    best = Schedule()
+    buf = C.create_string_buffer(8192)
+    rc = lib().scft_autotune(C.byref(p), Wp, int(reps), C.byref(best), buf, len(buf))
+    if rc != SCFT_OK:
+        raise ScftError(rc, "scft_autotune failed")
+    return best.as_dict(), buf.value.decode()
+
+
 class Context:
-    """One scft_ctx: grid, chain parameters, nrep replicas' device state."""
+    """One scft_ctx: grid, chain parameters, nrep replicas' device state. schedule: None (the
+    defaults), a dict as returned by autotune(), or a Schedule."""
 
     def __init__(self, m, D, N, f, alpha, ens, phidb, chi, nrep=1, precision="fp64", fix=(0, 0),
                  dostring=1, justFE=0, springsteps=0, device=0, fe_fp64=1, phihA=0.0, anderson=0,
-                 anderson_lambda=0.0, anderson_start=0.0, autotune=0, ckpt=0):
+                 anderson_lambda=0.0, anderson_start=0.0, autotune=0, ckpt=0, schedule=None):
         self.m = tuple(int(v) for v in m)
         self.D = tuple(float(v) for v in D)
         self.M = int(np.prod(self.m))
@@ -148,7 +198,8 @@ class Context:
                    precision=prec, fix=(C.c_int * 2)(*fix), dostring=int(dostring), justFE=int(justFE),
                    springsteps=int(springsteps), device=int(device), fe_fp64=int(fe_fp64),
                    phihA=float(phihA), anderson=int(anderson), anderson_lambda=float(anderson_lambda),
-                   anderson_start=float(anderson_start), autotune=int(autotune), ckpt=int(ckpt))
+                   anderson_start=float(anderson_start), autotune=int(autotune), ckpt=int(ckpt),
+                   sched=make_schedule(schedule))
         self.params = p
         h = C.c_void_p()
         rc = lib().scft_ctx_create(C.byref(p), C.byref(h))

@@ -27,7 +27,7 @@ struct KS {
     // 32 for x-length 48 and 96 (96^3 FP64 0.977 -> 0.800 ms, FP32 -5%, 48^3 -3%; 16 is no
     // better), 64 elsewhere (64^3 is 1.5-2.5% slower with 32, 128^3 unchanged).
     static constexpr int TC = (N0 == 48 || N0 == 96) ? 32 : 64;
-    // the other width, for the two-stage column kernel: the autotuner may pick it
+    // the other width, for the two-stage column kernel (a schedule may pick it)
     static constexpr int TC_ALT = TC == 64 ? 32 : 64;
     static constexpr int NTC = 256;
     static constexpr size_t lds_plane = (size_t)(N1 * PlaneGeom<N1, N2>::P2 + N1 + N2) * sizeof(cpx<T>);
@@ -46,6 +46,10 @@ struct KS {
     static constexpr size_t lds2_col_alt = (size_t)(CGA::LDSN + N0) * sizeof(cpx<T>) + (size_t)N0 * sizeof(T);
     static constexpr bool alt_ok = two_ok && CGA::ok && lds2_col_alt <= 160 * 1024;
     static constexpr int ntile_alt = (N1 * N2 + TC_ALT - 1) / TC_ALT;
+    // the other plane workgroup size, for planes of 512 or 768 threads (64^2, 96^2): half as
+    // many threads, each with twice the items
+    static constexpr int NT_ALT = (PG::NT >= 512 && PG::NT < 1024) ? PG::NT / 2 : 0;
+    static constexpr bool palt_ok = two_ok && NT_ALT > 0;
     static_assert(legacy_ok || two_ok, "no kernel fits in LDS for this grid");
     static_assert(fft_len_supported(N0) && fft_len_supported(N1) && fft_len_supported(N2), "radix");
 
@@ -108,6 +112,14 @@ struct KS {
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2_col_alt);
                 if (e != hipSuccess) return e;
             }
+            if constexpr (palt_ok) {
+                e = hipFuncSetAttribute((const void*)k_plane_end2<T, N1, N2, NT_ALT>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2_plane);
+                if (e != hipSuccess) return e;
+                e = hipFuncSetAttribute((const void*)k_plane_mid2<T, N1, N2, NT_ALT>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2_plane);
+                if (e != hipSuccess) return e;
+            }
             return hipFuncSetAttribute((const void*)k_column2<T, N0, N1, N2, TC>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2_col);
         }
@@ -126,37 +138,53 @@ struct KS {
         }
         hipLaunchKernelGGL((k_column2<T, N0, N1, N2, TC>), dim3(ntile, rows), dim3(CG::NT), lds2_col, st, a, a.nb, nf);
     }
+    // plane_end / plane_mid at the workgroup size a.planent asks for (0: PG::NT)
+    static void plane_end_two(const PropArgs<T>& a, int s, int mode, hipStream_t st)
+    {
+        if constexpr (palt_ok) {
+            if (a.planent == NT_ALT) {
+                hipLaunchKernelGGL((k_plane_end2<T, N1, N2, NT_ALT>), dim3(N0, a.nb), dim3(NT_ALT), lds2_plane, st,
+                                   a, s, mode);
+                return;
+            }
+        }
+        hipLaunchKernelGGL((k_plane_end2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, s, mode);
+    }
+    static void plane_mid_two(const PropArgs<T>& a, int s, hipStream_t st)
+    {
+        if constexpr (palt_ok) {
+            if (a.planent == NT_ALT) {
+                hipLaunchKernelGGL((k_plane_mid2<T, N1, N2, NT_ALT>), dim3(N0, a.nb), dim3(NT_ALT), lds2_plane, st,
+                                   a, s);
+                return;
+            }
+        }
+        if constexpr (persist_ok) {
+            if (a.pgrid > 0) {
+                const int np = N0 * a.nb;
+                hipLaunchKernelGGL((k_plane_mid2p<T, N1, N2>), dim3(std::min(a.pgrid, np)), dim3(PG::NT), lds2_plane,
+                                   st, a, s, np);
+                return;
+            }
+        }
+        hipLaunchKernelGGL((k_plane_mid2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, s);
+    }
     static void begin_two(const PropArgs<T>& a, hipStream_t st)
     {
-        if constexpr (two_ok)
-            hipLaunchKernelGGL((k_plane_end2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, 0,
-                               (int)PL_BEGIN);
+        if constexpr (two_ok) plane_end_two(a, 0, (int)PL_BEGIN, st);
     }
     static void step_two(const PropArgs<T>& a, int s, hipStream_t st)
     {
         if constexpr (two_ok) {
             column_two(a, 2 * a.nb, a.nb, st);
-            bool persistent = false;
-            if constexpr (persist_ok) {
-                if (a.pgrid > 0) {
-                    const int np = N0 * a.nb;
-                    hipLaunchKernelGGL((k_plane_mid2p<T, N1, N2>), dim3(std::min(a.pgrid, np)), dim3(PG::NT),
-                                       lds2_plane, st, a, s, np);
-                    persistent = true;
-                }
-            }
-            if (!persistent)
-                hipLaunchKernelGGL((k_plane_mid2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, s);
+            plane_mid_two(a, s, st);
             column_two(a, a.nb, 0, st);
-            hipLaunchKernelGGL((k_plane_end2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, s,
-                               (int)(s == a.N - 1 ? PL_LAST : PL_STEP));
+            plane_end_two(a, s, (int)(s == a.N - 1 ? PL_LAST : PL_STEP), st);
         }
     }
     static void resume_two(const PropArgs<T>& a, int c, hipStream_t st)
     {
-        if constexpr (two_ok)
-            hipLaunchKernelGGL((k_plane_end2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, c,
-                               (int)PL_RESUME);
+        if constexpr (two_ok) plane_end_two(a, c, (int)PL_RESUME, st);
     }
     static hipError_t gsetup_two()
     {
@@ -193,6 +221,8 @@ struct KS {
         KernelSet<T> k{N0,      N1,      N2,      &setup_legacy, &begin_legacy,  &step_legacy,
                        nullptr, nullptr, nullptr, "stockham",    &resume_legacy, nullptr,
                        lds_plane > 80 * 1024, TC, alt_ok ? TC_ALT : 0, persist_ok, nullptr, nullptr};
+        k.plane_nt = PG::NT;
+        k.plane_nt_alt = palt_ok ? NT_ALT : 0;
         if (two_ok) {
             k.setup2 = &setup_two;
             k.begin2 = &begin_two;
@@ -415,7 +445,8 @@ static hipError_t upload_twiddles(cpx<T>** dst, int n)
 }
 
 template <typename T>
-std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nbmax, int ck)
+std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nbmax, int ck,
+                                const scft_schedule* sched)
 {
     for (int i = 0; i < 3; ++i) { m_[i] = m[i]; D_[i] = D[i]; }
     N_ = N;
@@ -423,14 +454,11 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
     ck_ = ck > 0 ? ck : 0;
     qpitch_ = ck_ ? ck_pitch(N, ck_) : N + 1;
     M_ = (long long)m[0] * m[1] * m[2];
-    // Plane/column kernels where this grid has them; the line kernels otherwise, or when
-    // SCFT_PENCIL=1 is set.
-    const char* pen = getenv("SCFT_PENCIL");
+    const scft_schedule sc = sched ? *sched : scft_schedule{0, 0, 0, 0};
     const char* pen_rt = getenv("SCFT_PENCIL_RT");  // =1: runtime-length line kernels only
     pencil_ct_ = !(pen_rt && pen_rt[0] == '1');
-    ks_ = (pen && pen[0] == '1') ? nullptr : find_kernels<T>(m[0], m[1], m[2]);
-    const char* legacy = getenv("SCFT_LEGACY_FFT");
-    const bool force_legacy = legacy && legacy[0] == '1';
+    const KernelSet<T>* kset = find_kernels<T>(m[0], m[1], m[2]);
+    const bool pencil_ok = pencil_supported(m[0], m[1], m[2]);
     // persistent plane_mid (grids whose kernel set has it): one workgroup per CU;
     // SCFT_PLANE_GRID sets the count, 0 turns it off
     {
@@ -440,37 +468,73 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
             pgrid_ = cus;
         if (const char* pg = getenv("SCFT_PLANE_GRID")) pgrid_ = std::max(0, atoi(pg));
     }
-    // Without a two-stage set, a Stockham plane that fills a CU's LDS runs one workgroup
-    // per CU; the line kernels are faster there (96^3 FP64: 0.529 against 0.805 ms per
-    // contour step for 8 replicas).
-    const KernelSet<T>* kfound = ks_;
-    if (ks_ && !ks_->step2 && ks_->big_plane && !force_legacy && pencil_supported(m[0], m[1], m[2])) ks_ = nullptr;
+    // Kernel family. Default: plane/column kernels where this grid has them (two-stage when
+    // it has that set), the line kernels otherwise or when SCFT_PENCIL=1; SCFT_LEGACY_FFT=1
+    // keeps the Stockham plane/column kernels. Without a two-stage set, a Stockham plane that
+    // fills a CU's LDS runs one workgroup per CU, and the line kernels are faster there (96^3
+    // FP64: 0.529 against 0.805 ms per contour step for 8 replicas).
+    int family = sc.family;
+    if (family == SCFT_SCHED_DEFAULT) {
+        const char* pen = getenv("SCFT_PENCIL");
+        const char* legacy = getenv("SCFT_LEGACY_FFT");
+        const bool force_legacy = legacy && legacy[0] == '1';
+        if ((pen && pen[0] == '1') || !kset) family = SCFT_SCHED_PENCIL;
+        else if (kset->step2 && !force_legacy) family = SCFT_SCHED_TWO_STAGE;
+        else if (!kset->step2 && kset->big_plane && !force_legacy && pencil_ok) family = SCFT_SCHED_PENCIL;
+        else family = SCFT_SCHED_STOCKHAM;
+    } else {
+        // an explicit family must exist as such for this grid
+        const bool have = family == SCFT_SCHED_TWO_STAGE ? kset && kset->step2
+                          : family == SCFT_SCHED_STOCKHAM ? kset && kset->step != kset->step2
+                          : family == SCFT_SCHED_PENCIL   ? pencil_ok
+                                                          : false;
+        if (!have) return "schedule: kernel family " + std::to_string(family) + " is not available for this grid";
+    }
     hipError_t e = hipSuccess;
-    if (ks_) {
-        const bool use_two = ks_->step2 && !force_legacy;
+    if (family == SCFT_SCHED_TWO_STAGE || family == SCFT_SCHED_STOCKHAM) {
+        ks_ = kset;
+        const bool use_two = family == SCFT_SCHED_TWO_STAGE;
         setup_ = use_two ? ks_->setup2 : ks_->setup;
         begin_ = use_two ? ks_->begin2 : ks_->begin;
         step_ = use_two ? ks_->step2 : ks_->step;
         resume_ = use_two ? ks_->resume2 : ks_->resume;
         path_ = use_two ? "two-stage" : "stockham";
-        // SCFT_COLTILE=<width>: the two-stage column pass at the other tile width
-        const char* ct = getenv("SCFT_COLTILE");
-        if (use_two && ct && ks_->tc_alt && atoi(ct) == ks_->tc_alt) {
-            coltile_ = ks_->tc_alt;
-            path_ = ks_->tc_alt == 32 ? "two-stage-c32" : "two-stage-c64";
+        // column tile: the schedule's, else SCFT_COLTILE=<width>, else the grid's default
+        int ct = sc.coltile;
+        if (ct == 0 && use_two) {
+            const char* cte = getenv("SCFT_COLTILE");
+            if (cte && ks_->tc_alt && atoi(cte) == ks_->tc_alt) ct = ks_->tc_alt;
         }
+        if (ct != 0 && ct != ks_->tc && !(use_two && ct == ks_->tc_alt))
+            return "schedule: column tile " + std::to_string(ct) + " is not available for this grid";
+        if (ct != 0 && ct != ks_->tc) {
+            coltile_ = ct;
+            path_ = ct == 32 ? "two-stage-c32" : "two-stage-c64";
+        }
+        // plane workgroup size: the schedule's, else SCFT_PLANE_NT=<threads>, else the default
+        int pn = sc.plane_nt;
+        if (pn == 0 && use_two) {
+            const char* pne = getenv("SCFT_PLANE_NT");
+            if (pne && ks_->plane_nt_alt && atoi(pne) == ks_->plane_nt_alt) pn = ks_->plane_nt_alt;
+        }
+        if (pn != 0 && !(use_two && (pn == ks_->plane_nt || pn == ks_->plane_nt_alt)))
+            return "schedule: plane workgroup size " + std::to_string(pn) + " is not available for this grid";
+        if (pn != 0 && pn != ks_->plane_nt) planent_ = pn;
         e = setup_();
-    } else if (pencil_supported(m[0], m[1], m[2])) {
+    } else {
+        if (!pencil_ok) {
+            char b[512];
+            snprintf(b, sizeof b, "no %s propagator kernels for grid %dx%dx%d (supported: %s)",
+                     sizeof(T) == 8 ? "FP64" : "FP32", m[0], m[1], m[2], supported_grids(sizeof(T) == 8).c_str());
+            return b;
+        }
+        if (sc.coltile != 0 || sc.plane_nt != 0) return "schedule: tile sizes apply to the two-stage kernels only";
+        ks_ = nullptr;
         begin_ = &pencil_begin<T>;
         step_ = &pencil_step<T>;
         resume_ = &pencil_resume<T>;
         path_ = "pencil";
         e = pencil_setup<T>();
-    } else {
-        char b[512];
-        snprintf(b, sizeof b, "no %s propagator kernels for grid %dx%dx%d (supported: %s)",
-                 sizeof(T) == 8 ? "FP64" : "FP32", m[0], m[1], m[2], supported_grids(sizeof(T) == 8).c_str());
-        return b;
     }
     if (e != hipSuccess) return std::string("hipFuncSetAttribute: ") + hipGetErrorString(e);
     nblk_ = (int)std::min<long long>(1024, (M_ + RED_NT - 1) / RED_NT);
@@ -504,104 +568,54 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
     // launches from two streams (0.0816 vs 0.0744 ms per contour step, 14 x 64^3 FP32).
     const char* gr = getenv("SCFT_GRAPH");
     use_graph_ = gr && gr[0] == '1';
+    // replica-group streams: the schedule's, else SCFT_STREAMS, else the default
+    if (sc.streams < 0 || sc.streams > kMaxStreams)
+        return "schedule: " + std::to_string(sc.streams) + " replica-group streams (1 .. " +
+               std::to_string(kMaxStreams) + ")";
     const char* ns = getenv("SCFT_STREAMS");
-    nstreams_ = std::max(1, std::min(kMaxStreams, ns ? atoi(ns) : kDefaultStreams));
+    nstreams_ = sc.streams > 0 ? sc.streams : std::max(1, std::min(kMaxStreams, ns ? atoi(ns) : kDefaultStreams));
     // fork/join events (no timing) and the extra streams this replica-group count uses
     if (hipEventCreateWithFlags(&fork_, hipEventDisableTiming) != hipSuccess) return "event creation failed";
     if (ensure_streams(nstreams_) != hipSuccess) return "stream / event creation failed";
-    // kernel families this grid can run, for the autotuner
-    fams_.clear();
-    if (kfound && kfound->step2) {
-        fams_.push_back({"two-stage", kfound->setup2, kfound->begin2, kfound->step2, kfound->resume2, 0});
-        // the same kernels with the other column tile width
-        if (kfound->tc_alt)
-            fams_.push_back({kfound->tc_alt == 32 ? "two-stage-c32" : "two-stage-c64", kfound->setup2, kfound->begin2,
-                             kfound->step2, kfound->resume2, kfound->tc_alt});
-    }
-    if (kfound && kfound->step != kfound->step2)
-        fams_.push_back({"stockham", kfound->setup, kfound->begin, kfound->step, kfound->resume, 0});
-    if (pencil_supported(m[0], m[1], m[2]))
-        fams_.push_back({"pencil", &pencil_setup<T>, &pencil_begin<T>, &pencil_step<T>, &pencil_resume<T>, 0});
     return "";
 }
 
-// Times every configuration of the contour loop on the batch at hand (same fields, nb, cc):
-// kernel family x replica-group streams (1 .. 3) x graph replay, best of two runs each with
-// HIP events on st, and keeps the fastest. The contour loop recomputes everything it
-// writes, so the trial runs leave no trace.
+// Every schedule this grid can run for nb slots: the kernel families, for the two-stage
+// kernels each column tile x plane workgroup size, each with 1 .. min(3, nb) replica-group
+// streams.
 template <typename T>
-hipError_t PropEngine<T>::tune(const PropArgs<T>& a, int nb, const ChainConst& cc, hipStream_t st)
+std::vector<scft_schedule> PropEngine<T>::candidates(const int m[3], int nb)
 {
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipError_t e = hipEventCreate(&e0);
-    if (e == hipSuccess) e = hipEventCreate(&e1);
-    struct Choice {
-        int fam, streams;
-        bool graph;
-    };
-    Choice best{-1, 1, false};
-    float best_ms = 0;
-    tune_log_.clear();
-    for (int f = 0; f < (int)fams_.size() && e == hipSuccess; ++f) {
-        if ((e = fams_[f].setup()) != hipSuccess) break;
-        for (int ns = 1; ns <= std::min(3, kMaxStreams) && e == hipSuccess; ++ns) {
-            if (ns > 1 && ns > nb) break;
-            if ((e = ensure_streams(ns)) != hipSuccess) break;
-            for (int gph = 0; gph < 2 && e == hipSuccess; ++gph) {
-                begin_ = fams_[f].begin;
-                step_ = fams_[f].step;
-                resume_ = fams_[f].resume;
-                coltile_ = fams_[f].coltile;
-                nstreams_ = ns;
-                use_graph_ = gph == 1;
-                clear_graphs();
-                float t = -1;
-                for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
-                    if ((e = hipEventRecord(e0, st)) != hipSuccess) break;
-                    PropArgs<T> at = a;
-                    at.coltile = coltile_;
-                    if ((e = contour(at, nb, cc, st)) != hipSuccess) break;
-                    if ((e = hipEventRecord(e1, st)) != hipSuccess) break;
-                    if ((e = hipEventSynchronize(e1)) != hipSuccess) break;
-                    float ms = 0;
-                    if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
-                    t = t < 0 ? ms : std::min(t, ms);
-                }
-                if (e != hipSuccess) break;
-                char line[128];
-                snprintf(line, sizeof line, "%s/%d/%s %.3f; ", fams_[f].name, ns, gph ? "graph" : "direct", t);
-                tune_log_ += line;
-                if (best.fam < 0 || t < best_ms) {
-                    best = {f, ns, gph == 1};
-                    best_ms = t;
-                }
-            }
-        }
+    std::vector<scft_schedule> out;
+    const KernelSet<T>* k = find_kernels<T>(m[0], m[1], m[2]);
+    std::vector<scft_schedule> fam;
+    if (k && k->step2) {
+        const int cts[2] = {k->tc, k->tc_alt}, pns[2] = {k->plane_nt, k->plane_nt_alt};
+        for (int ct : cts)
+            for (int pn : pns)
+                if (ct && pn) fam.push_back({SCFT_SCHED_TWO_STAGE, ct, pn, 0});
     }
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    if (e != hipSuccess || best.fam < 0) return e != hipSuccess ? e : hipErrorUnknown;
-    if ((e = fams_[best.fam].setup()) != hipSuccess) return e;
-    begin_ = fams_[best.fam].begin;
-    step_ = fams_[best.fam].step;
-    resume_ = fams_[best.fam].resume;
-    coltile_ = fams_[best.fam].coltile;
-    path_ = fams_[best.fam].name;
-    nstreams_ = best.streams;
-    use_graph_ = best.graph;
-    clear_graphs();
-    tuned_ = true;
-    return hipSuccess;
+    if (k && k->step != k->step2) fam.push_back({SCFT_SCHED_STOCKHAM, 0, 0, 0});
+    if (pencil_supported(m[0], m[1], m[2])) fam.push_back({SCFT_SCHED_PENCIL, 0, 0, 0});
+    for (scft_schedule f : fam)
+        for (int ns = 1; ns <= std::min(3, std::max(1, nb)); ++ns) {
+            f.streams = ns;
+            out.push_back(f);
+        }
+    return out;
 }
 
 template <typename T>
 std::string PropEngine<T>::config() const
 {
     char b[320];
-    int n = snprintf(b, sizeof b, "%s, %d replica-group stream%s, %s%s", path_, nstreams_, nstreams_ > 1 ? "s" : "",
-                     use_graph_ ? "graph replay" : "direct launches", tuned_ ? " (autotuned)" : "");
-    if (ks_ && ks_->persist_mid && pgrid_ > 0 && strncmp(path_, "two-stage", 9) == 0 && n > 0 && n < (int)sizeof b)
+    int n = snprintf(b, sizeof b, "%s, %d replica-group stream%s, %s", path_, nstreams_, nstreams_ > 1 ? "s" : "",
+                     use_graph_ ? "graph replay" : "direct launches");
+    if (ks_ && ks_->persist_mid && pgrid_ > 0 && planent_ == 0 && strncmp(path_, "two-stage", 9) == 0 && n > 0 &&
+        n < (int)sizeof b)
         n += snprintf(b + n, sizeof b - n, ", persistent plane_mid on %d workgroups", pgrid_);
+    if (planent_ > 0 && n > 0 && n < (int)sizeof b)
+        n += snprintf(b + n, sizeof b - n, ", %d-thread plane workgroups", planent_);
     if (ck_ && n > 0 && n < (int)sizeof b)
         snprintf(b + n, sizeof b - n, ", q checkpointed every %d steps (%d of %d entries per replica)", ck_, qpitch_,
                  N_ + 1);
@@ -641,12 +655,6 @@ hipError_t PropEngine<T>::run(const double* dW, long long wstride, double* dphi,
     hipLaunchKernelGGL(k_wmean, dim3(nb), dim3(RED_NT), 0, st, part_, nblk_, M_, cAB_);
     hipLaunchKernelGGL((k_factors<T>), rg, dim3(RED_NT), 0, st, dW, wstride, d_slot2rep, M_, cc.ds, cAB_, e4A_,
                        e4B_);
-    if (tune_pending_) {
-        tune_pending_ = false;
-        hipError_t et = tune(a, nb, cc, st);
-        if (et != hipSuccess) return et;
-        a = args(nb, cc);  // the chosen configuration
-    }
     if (timing_) (void)hipEventRecord(ev_[0], st);
     hipError_t e = contour(a, nb, cc, st);
     if (e != hipSuccess) return e;
@@ -726,6 +734,7 @@ PropArgs<T> PropEngine<T>::args(int nb, const ChainConst& cc) const
     a.pencil_ct = pencil_ct_ ? 1 : 0;
     a.coltile = coltile_;
     a.pgrid = pgrid_;
+    a.planent = planent_;
     a.Ch = Ch_;
     a.Cf = Cf_;
     a.e4A = e4A_;

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "prop_types.hpp"
+#include "scft_api.h"
 
 namespace scft {
 
@@ -36,6 +37,8 @@ struct KernelSet {
     hipError_t (*gsetup2)();
     void (*gterm2)(const PropArgs<T>&, int ib, int pbk, int ia, int pak, int spec, cpx<T>* K0, cpx<T>* K1,
                    cpx<T>* Kf, hipStream_t);
+    // two-stage plane kernels: the default workgroup size and the alternative (0: none)
+    int plane_nt, plane_nt_alt;
 };
 
 // kernel set for an (m0, m1, m2) grid, or nullptr when no instantiation exists
@@ -56,7 +59,12 @@ public:
     // Allocates buffers for nbmax slots. ck = 0 keeps q at every contour index; ck = K > 0
     // keeps every K-th (ck_index) and recomputes the rest in windows (run, box_gradient).
     // Returns "" on success, else an error message.
-    std::string init(const int m[3], const double D[3], int N, int nbmax, int ck = 0);
+    // sched (nullptr: the defaults) picks the kernel family, tile sizes and streams; a
+    // choice this grid cannot run is an error ("schedule: ...").
+    std::string init(const int m[3], const double D[3], int N, int nbmax, int ck = 0,
+                     const scft_schedule* sched = nullptr);
+    // every schedule the grid can run for nb slots (the autotuner's candidates)
+    static std::vector<scft_schedule> candidates(const int m[3], int nb);
     // Recomputes the kinetic factor tables for a new box D.
     hipError_t set_box(const double D[3]);
 
@@ -82,13 +90,8 @@ public:
     int nbmax() const { return nbmax_; }
     size_t bytes_allocated() const { return bytes_; }
     const char* fft_path() const { return path_; }
-    // The next run() first times every kernel family x stream count x graph replay on its
-    // batch and keeps the fastest (see tune()).
-    void request_autotune() { tune_pending_ = true; }
-    // "two-stage, 2 replica-group streams, direct launches (autotuned)"
+    // "two-stage, 2 replica-group streams, direct launches"
     std::string config() const;
-    // per-candidate timings of the last autotune ("family/streams/mode ms; ...")
-    const std::string& tune_log() const { return tune_log_; }
     // HIP events around the contour loop of the next run() calls (diagnostics)
     hipError_t enable_timing(bool on);
     // elapsed ms of the contour loop of the last run() (after the stream is synchronised)
@@ -98,18 +101,6 @@ private:
     static constexpr int kMaxStreams = 4;
     static constexpr int kDefaultStreams = 2;
     PropArgs<T> args(int nb, const ChainConst& cc) const;
-    hipError_t tune(const PropArgs<T>& a, int nb, const ChainConst& cc, hipStream_t st);
-    struct Family {
-        const char* name;
-        hipError_t (*setup)();
-        void (*begin)(const PropArgs<T>&, hipStream_t);
-        void (*step)(const PropArgs<T>&, int s, hipStream_t);
-        void (*resume)(const PropArgs<T>&, int c, hipStream_t);
-        int coltile;  // two-stage column tile width (0: the grid's default)
-    };
-    std::vector<Family> fams_;
-    bool tune_pending_ = false, tuned_ = false;
-    std::string tune_log_;
     hipError_t ensure_streams(int n);
     hipError_t contour(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st);
     // fork / join: the events that order the replica-group streams after st and st after
@@ -149,6 +140,7 @@ private:
     int ck_ = 0;      // checkpoint interval (0: every contour index kept)
     int coltile_ = 0;  // two-stage column tile width (0: the grid's default)
     int pgrid_ = 0;    // workgroups of the persistent plane kernels (0: one per plane)
+    int planent_ = 0;  // two-stage plane workgroup size (0: the grid's default)
     int qpitch_ = 0;  // q-store entries per slot
     const char* path_ = "";
     cpx<T>* Ch_ = nullptr;

@@ -64,11 +64,14 @@ constexpr int split_wlz(int N)
 //   Z2 (y, k2)  holds Z[y][k2 + A2 k1], k1 < A1      (after the z transform)
 //   Y1 (kz, b1) holds P[b1 + B1 b2][kz], b2 < B2
 //   Y2 (kz, l2) holds Y[l2 + B2 l1][kz], l1 < B1     (k-space ownership, global layout)
-template <int NY, int NZ>
+// NTO > 0 sets the workgroup size (a multiple of 64); 0 takes one thread per Z1 item.
+template <int NY, int NZ, int NTO = 0>
 struct P2Geom {
     static constexpr int A1 = split_wlz(NZ), A2 = NZ / A1;
     static constexpr int B1 = split_n1(NY), B2 = NY / B1;
-    static constexpr int NT = cmax(64, cdiv(NY * A1, 64) * 64) > 1024 ? 1024 : cmax(64, cdiv(NY * A1, 64) * 64);
+    static constexpr int NT0 = cmax(64, cdiv(NY * A1, 64) * 64) > 1024 ? 1024 : cmax(64, cdiv(NY * A1, 64) * 64);
+    static constexpr int NT = NTO > 0 ? NTO : NT0;
+    static_assert(NT % 64 == 0 && NT <= 1024, "plane workgroup size");
     static constexpr int IZ1 = cdiv(NY * A1, NT), IZ2 = cdiv(NY * A2, NT);
     static constexpr int IY1 = cdiv(NZ * B1, NT), IY2 = cdiv(NZ * B2, NT);
     // Wave-local z transposes: when A1 divides A2 and 64, each stage-1 item (y, a1) also
@@ -86,9 +89,9 @@ struct P2Geom {
     static constexpr bool ok = A1 > 1 && B1 > 1;
 };
 
-template <typename T, int NY, int NZ>
-struct P2 : P2Geom<NY, NZ> {
-    using G = P2Geom<NY, NZ>;
+template <typename T, int NY, int NZ, int NTO = 0>
+struct P2 : P2Geom<NY, NZ, NTO> {
+    using G = P2Geom<NY, NZ, NTO>;
     using G::A1; using G::A2; using G::B1; using G::B2; using G::NT;
     using G::IZ1; using G::IY1; using G::IY2;
     using G::PZ; using G::RZ; using G::PP; using G::PY; using G::CY; using G::WLZ; using G::IZ2E;
@@ -364,16 +367,16 @@ __device__ __forceinline__ void p2_load_tw(cpx<T>* twz, cpx<T>* twy, const PropA
 // 64^2 plane otherwise takes 132 VGPRs, one 512-thread workgroup per CU; capped at 4 waves it
 // takes 126 with no spill and two workgroups fit (64^3 FP64 x 14: 0.1456 -> 0.1367 ms per
 // contour step, bit-identical results).
-template <typename T, int NE>
-constexpr int end_wpe() { return (NE == 4096 && sizeof(T) == 8) ? 4 : 1; }
+template <typename T, int NE, int NT>
+constexpr int end_wpe() { return (NE == 4096 && NT == 512 && sizeof(T) == 8) ? 4 : 1; }
 
 // plane_end on the two-stage FFT (see k_plane_end in prop_kernels.hpp for the contract)
-template <typename T, int NY, int NZ>
-__global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT),
-                               amdgpu_waves_per_eu(end_wpe<T, NY * NZ>())))
+template <typename T, int NY, int NZ, int NTO = 0>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ, NTO>::NT),
+                               amdgpu_waves_per_eu(end_wpe<T, NY * NZ, P2Geom<NY, NZ, NTO>::NT>())))
 k_plane_end2(PropArgs<T> a, int s, int mode)
 {
-    using P = P2<T, NY, NZ>;
+    using P = P2<T, NY, NZ, NTO>;
     constexpr int NT = P::NT, IZ1 = P::IZ1, A2 = P::A2, NE = NY * NZ;
     extern __shared__ unsigned char smem_raw[];
     cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
@@ -526,10 +529,11 @@ k_gplane_init2(PropArgs<T> a, int ib, int pbk, int ia, int pak, int spec)
 }
 
 // plane_mid on the two-stage FFT: Ch <- F2( E1/2 * F2^-1(Ch) )
-template <typename T, int NY, int NZ>
-__global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT))) k_plane_mid2(PropArgs<T> a, int s)
+template <typename T, int NY, int NZ, int NTO = 0>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ, NTO>::NT)))
+k_plane_mid2(PropArgs<T> a, int s)
 {
-    using P = P2<T, NY, NZ>;
+    using P = P2<T, NY, NZ, NTO>;
     constexpr int NT = P::NT, IZ1 = P::IZ1, A2 = P::A2, NE = NY * NZ;
     extern __shared__ unsigned char smem_raw[];
     cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);

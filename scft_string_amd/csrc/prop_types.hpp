@@ -16,6 +16,7 @@ struct PropArgs {
     int coltile;           // two-stage column pass: tile width (0: the grid's default)
     int pgrid;             // planes with a persistent plane_mid: > 0 runs it on that many
                            // workgroups; 0 launches one workgroup per plane
+    int planent;           // two-stage plane kernels: workgroup size (0: the grid's default)
     cpx<T>* Ch;            // [slot][M] half-step sandwich work field
     cpx<T>* Cf;            // [slot][M] full-step sandwich work field
     const T* e4A;          // [slot][M] quarter-step factor of species A (Boltz<T> form)

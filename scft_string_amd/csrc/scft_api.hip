@@ -29,6 +29,7 @@ struct scft_ctx {
     hipStream_t st = nullptr;
     std::string err;
     std::string info;  // scft_engine_info
+    std::string tuned;  // the autotuner's candidate timings when autotune = 1 chose the schedule
     std::unique_ptr<PropEngine<float>> pe32;
     std::unique_ptr<PropEngine<double>> pe64;
     // FP32 contexts: FP64 engine for the props that FreeE takes F from (created on first use)
@@ -352,6 +353,17 @@ scft_status anderson_iterate(scft_ctx* c, double* alf, scft_iter_report* rep)
     return SCFT_OK;
 }
 
+// "two-stage c64 nt512 s2", "stockham s1", "pencil s3"
+std::string sched_name(const scft_schedule& s)
+{
+    char b[96];
+    if (s.family == SCFT_SCHED_TWO_STAGE)
+        snprintf(b, sizeof b, "two-stage c%d nt%d s%d", s.coltile, s.plane_nt, s.streams);
+    else
+        snprintf(b, sizeof b, "%s s%d", s.family == SCFT_SCHED_STOCKHAM ? "stockham" : "pencil", s.streams);
+    return b;
+}
+
 }  // namespace
 
 extern "C" {
@@ -368,10 +380,8 @@ const char* scft_fft_path(const scft_ctx* ctx)
 const char* scft_engine_info(scft_ctx* ctx)
 {
     if (!ctx) return "";
-    if (ctx->pe32) ctx->info = ctx->pe32->config() + (ctx->pe32->tune_log().empty() ? "" : "; tuned: ") +
-                               ctx->pe32->tune_log();
-    else if (ctx->pe64) ctx->info = ctx->pe64->config() + (ctx->pe64->tune_log().empty() ? "" : "; tuned: ") +
-                                    ctx->pe64->tune_log();
+    ctx->info = ctx->pe32 ? ctx->pe32->config() : ctx->pe64 ? ctx->pe64->config() : std::string();
+    if (!ctx->tuned.empty()) ctx->info += " (autotuned); candidates: " + ctx->tuned;
     return ctx->info.c_str();
 }
 
@@ -406,12 +416,19 @@ scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
     if (p->ens == 1 && p->phidb + p->phihA > 1.0) return SCFT_ERR_INVALID;
     if (p->anderson < 0 || p->anderson > 40 || (p->anderson > 0 && p->dostring != 0)) return SCFT_ERR_INVALID;
     if (p->ckpt < 0) return SCFT_ERR_INVALID;
-    // The autotuner (PropEngine::tune) is switched off: autotuned contexts crashed on the host
-    // (a segmentation fault in the first props) with the cause not found. Refused before any
-    // device work, so the rest of the library is unaffected.
+    const scft_schedule& sc = p->sched;
+    if (sc.family < SCFT_SCHED_DEFAULT || sc.family > SCFT_SCHED_PENCIL || sc.coltile < 0 || sc.plane_nt < 0 ||
+        sc.streams < 0 || sc.streams > 4)
+        return SCFT_ERR_INVALID;
+    // autotune = 1: the schedule comes from scft_autotune, which times every candidate in a
+    // context of its own before this one allocates anything
     if (p->autotune) {
-        fprintf(stderr, "scft_ctx_create: autotune = 1 is disabled in this build (host crash, cause not found)\n");
-        return SCFT_ERR_UNSUPPORTED;
+        scft_params q = *p;
+        q.autotune = 0;
+        char rep[4096];
+        const scft_status ts = scft_autotune(&q, nullptr, 2, &c->p.sched, rep, sizeof rep);
+        if (ts != SCFT_OK) return ts;
+        c->tuned = rep;
     }
 
     if (hipSetDevice(p->device) != hipSuccess) return SCFT_ERR_HIP;
@@ -419,21 +436,19 @@ scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
     std::string msg;
     if (p->precision == SCFT_FP32) {
         c->pe32 = std::make_unique<PropEngine<float>>();
-        msg = c->pe32->init(p->m, p->D, p->N, c->R, p->ckpt);
+        msg = c->pe32->init(p->m, p->D, p->N, c->R, p->ckpt, &c->p.sched);
     } else {
         c->pe64 = std::make_unique<PropEngine<double>>();
-        msg = c->pe64->init(p->m, p->D, p->N, c->R, p->ckpt);
-    }
-    if (msg.empty() && p->autotune) {
-        if (c->pe32) c->pe32->request_autotune();
-        else c->pe64->request_autotune();
+        msg = c->pe64->init(p->m, p->D, p->N, c->R, p->ckpt, &c->p.sched);
     }
     if (!msg.empty()) {
         fprintf(stderr, "scft_ctx_create: %s\n", msg.c_str());
         (void)hipStreamDestroy(c->st);
         c->st = nullptr;
         (void)hipGetLastError();  // a failed allocation must not surface in a later context's launch check
-        return msg.find("no ") == 0 ? SCFT_ERR_UNSUPPORTED : SCFT_ERR_NOMEM;
+        return msg.find("schedule:") == 0 ? SCFT_ERR_INVALID
+               : msg.find("no ") == 0     ? SCFT_ERR_UNSUPPORTED
+                                          : SCFT_ERR_NOMEM;
     }
     const long long M = c->M;
     const int R = c->R;
@@ -1148,6 +1163,91 @@ scft_status scft_debug_qq_invariant(scft_ctx* c, int slot, double* qq)
     SCFTCHK(bind_device(c));
     hipError_t e = c->pe32 ? c->pe32->qq_invariant(slot, qq, c->st) : c->pe64->qq_invariant(slot, qq, c->st);
     if (e != hipSuccess) return fail(c, SCFT_ERR_HIP, hipGetErrorString(e));
+    return SCFT_OK;
+}
+
+scft_status scft_autotune(const scft_params* p, const double* W, int reps, scft_schedule* best, char* report,
+                          size_t report_len)
+{
+    if (!p || !best || reps < 1) return SCFT_ERR_INVALID;
+    if (report && report_len) report[0] = '\0';
+    if (p->m[0] <= 0 || p->m[1] <= 0 || p->m[2] <= 0 || p->nrep <= 0) return SCFT_ERR_INVALID;
+    if (p->precision != SCFT_FP64 && p->precision != SCFT_FP32) return SCFT_ERR_INVALID;
+    // the rows a string iteration propagates: every replica but the fixed ends
+    int nact = p->nrep;
+    if (p->dostring && !p->justFE) nact -= (p->fix[0] ? 1 : 0) + (p->fix[1] ? 1 : 0);
+    nact = std::max(1, nact);
+    const std::vector<scft_schedule> cand = p->precision == SCFT_FP32 ? PropEngine<float>::candidates(p->m, nact)
+                                                                       : PropEngine<double>::candidates(p->m, nact);
+    if (cand.empty()) return SCFT_ERR_UNSUPPORTED;
+    const long long M = (long long)p->m[0] * p->m[1] * p->m[2];
+    std::vector<double> trial;
+    const double* Wt = W;
+    if (!Wt) {
+        // lamellar W- along x with a small per-replica modulation along y, W+ = 0
+        trial.assign((size_t)p->nrep * 2 * (size_t)M, 0.0);
+        const double tpi = 8 * atan(1.0);
+        for (int i = 0; i < p->nrep; ++i)
+            for (long long r = 0; r < M; ++r) {
+                const long long x = r / ((long long)p->m[1] * p->m[2]), y = (r / p->m[2]) % p->m[1];
+                trial[(size_t)i * 2 * M + r] = 1.5 * cos(tpi * x / p->m[0]) +
+                                               0.1 * sin(tpi * (double)y / p->m[1] + tpi * i / p->nrep);
+            }
+        Wt = trial.data();
+    }
+    // one candidate in a context of its own: an untimed props, then the best of n timed ones
+    auto time_sched = [&](const scft_schedule& sc, int n, double* ms) -> scft_status {
+        scft_params q = *p;
+        q.autotune = 0;
+        q.sched = sc;
+        scft_ctx* c = nullptr;
+        scft_status st = scft_ctx_create(&q, &c);
+        if (st == SCFT_OK) st = scft_set_fields(c, Wt);
+        double t = 0;
+        if (st == SCFT_OK) st = scft_debug_time_props(c, nact, 1, &t);
+        for (int r = 0; r < n && st == SCFT_OK; ++r) {
+            st = scft_debug_time_props(c, nact, 1, &t);
+            if (st == SCFT_OK && (*ms < 0 || t < *ms)) *ms = t;
+        }
+        if (c) (void)scft_ctx_destroy(c);
+        return st;
+    };
+    std::vector<double> ms(cand.size(), -1.0);
+    std::vector<scft_status> sts(cand.size(), SCFT_OK);
+    for (size_t i = 0; i < cand.size(); ++i) sts[i] = time_sched(cand[i], reps, &ms[i]);
+    // Candidates within 3% of the fastest are timed again, each in a fresh context: the first
+    // pass separates schedules that differ by more than run-to-run noise, the second picks
+    // among the near ties.
+    double best_ms = -1;
+    for (size_t i = 0; i < cand.size(); ++i)
+        if (sts[i] == SCFT_OK && (best_ms < 0 || ms[i] < best_ms)) best_ms = ms[i];
+    if (best_ms > 0)
+        for (size_t i = 0; i < cand.size(); ++i)
+            if (sts[i] == SCFT_OK && ms[i] <= 1.03 * best_ms) sts[i] = time_sched(cand[i], reps + 1, &ms[i]);
+    std::string rep;
+    scft_status last = SCFT_OK;
+    best_ms = -1;
+    for (size_t i = 0; i < cand.size(); ++i) {
+        char b[160];
+        if (sts[i] == SCFT_OK) {
+            snprintf(b, sizeof b, "%s %.4f; ", sched_name(cand[i]).c_str(), ms[i]);
+            if (best_ms < 0 || ms[i] < best_ms) {
+                best_ms = ms[i];
+                *best = cand[i];
+            }
+        } else {
+            snprintf(b, sizeof b, "%s failed (status %d); ", sched_name(cand[i]).c_str(), (int)sts[i]);
+            last = sts[i];
+        }
+        rep += b;
+    }
+    if (best_ms < 0) return last != SCFT_OK ? last : SCFT_ERR_UNSUPPORTED;
+    rep += "ms per contour step over " + std::to_string(nact) + " rows; chosen: " + sched_name(*best);
+    if (report && report_len) {
+        const size_t n = std::min(report_len - 1, rep.size());
+        std::memcpy(report, rep.data(), n);
+        report[n] = '\0';
+    }
     return SCFT_OK;
 }
 

@@ -273,3 +273,29 @@ def test_column_tile_widths_agree(m, prec, width, monkeypatch):
             assert ctx.fft_path == ("two-stage" if ct == "0" else f"two-stage-c{width}")
             out[ct] = ctx.props_batch(W)
     assert np.array_equal(out["0"][0], out[str(width)][0]) and np.array_equal(out["0"][1], out[str(width)][1])
+
+
+@pytest.mark.parametrize("m,prec,nt", [((64, 64, 64), "fp32", 256), ((64, 64, 64), "fp64", 256),
+                                       ((96, 96, 96), "fp32", 384)])
+def test_plane_workgroup_sizes_agree(m, prec, nt, monkeypatch):
+    """The two-stage plane kernels at half the default workgroup size (SCFT_PLANE_NT; each
+    thread then takes two items of every stage) run the same transforms on the same data:
+    props agree to rounding (the compiler may contract differently per instantiation), and
+    the symmetric-operator invariant holds."""
+    D = (6.0, 6.0, 6.0)
+    W = np.stack([_fields(m, D, seed=40 + i, amp=1.5) for i in range(2)])
+    out = {}
+    for pn in ("0", str(nt)):
+        monkeypatch.setenv("SCFT_PLANE_NT", pn)
+        with Context(m, D, 24, 0.45, 0.1, 1, 0.9, chi=18.0, nrep=2, precision=prec) as ctx:
+            assert ("%d-thread plane workgroups" % nt in ctx.engine_info) == (pn != "0"), ctx.engine_info
+            phi, lnQ, _ = ctx.props_batch(W)
+            qq = ctx.qq_invariant(1)
+        out[pn] = (phi, lnQ, qq)
+    tol = 1e-12 if prec == "fp64" else 2e-5
+    dl = float(np.max(np.abs(out["0"][1] - out[str(nt)][1])))
+    dphi = _rel(out[str(nt)][0], out["0"][0])
+    qq = out[str(nt)][2]
+    dev = float(np.max(np.abs(qq - qq[0])) / abs(qq[0]))
+    _say(f"{m} {prec} plane workgroups {nt} vs default: |dlnQ| {dl:.2e}, rel dphi {dphi:.2e}, invariant {dev:.2e}")
+    assert dl <= tol and dphi <= tol and dev <= (1e-12 if prec == "fp64" else 1e-5)
