@@ -426,17 +426,25 @@ def uneven_specs(with_string, precision="fp32"):
     return specs
 
 
-def uneven_contexts(with_string, precision="fp32"):
+def uneven_contexts(with_string, precision="fp32", tuned=False):
+    """The uneven batch's contexts with their initial fields. tuned: each context on the
+    schedule scft_autotune picks for it, timed alone on its own fields (returned too)."""
     from scft_string_amd import Context
+    from scft_string_amd.capi import autotune
 
-    ctxs = []
+    ctxs, scheds = [], []
     for kw, W in uneven_specs(with_string, precision):
         kw = dict(kw)
-        c = Context(kw.pop("m"), kw.pop("D"), kw.pop("N"), kw.pop("f"), kw.pop("alpha"), kw.pop("ens"),
-                    kw.pop("phidb"), kw.pop("chi"), **kw)
+        pos = [kw.pop(k) for k in ("m", "D", "N", "f", "alpha", "ens", "phidb", "chi")]
+        sched = None
+        if tuned:
+            sched, _ = autotune(*pos, nrep=kw["nrep"], precision=kw["precision"], fix=kw["fix"],
+                                dostring=kw["dostring"], W=W, reps=1)
+            scheds.append(sched)
+        c = Context(*pos, schedule=sched, **kw)
         c.set_fields(W)
         ctxs.append(c)
-    return ctxs
+    return (ctxs, scheds) if tuned else ctxs
 
 
 def uneven_report(K):
@@ -452,11 +460,17 @@ def uneven_report(K):
 
     solves = uneven_solves()
 
-    def build(with_string):
-        return uneven_contexts(with_string)
+    tuned_scheds = {}
 
-    def run(with_string, concurrent):
-        ctxs = build(with_string)
+    def build(with_string, tuned):
+        if not tuned:
+            return uneven_contexts(with_string)
+        ctxs, scheds = uneven_contexts(with_string, tuned=True)
+        tuned_scheds[with_string] = scheds
+        return ctxs
+
+    def run(with_string, concurrent, tuned=False):
+        ctxs = build(with_string, tuned)
         if concurrent:
             iterate_concurrent(ctxs, [0] * len(ctxs))
         else:
@@ -478,9 +492,12 @@ def uneven_report(K):
     for with_string in (False, True):
         tc, pc = run(with_string, True)
         ts, ps = run(with_string, False)
+        tt, pt = run(with_string, True, tuned=True)
         key = "string_24x128cube_N160_plus_12_solves" if with_string else "12_independent_solves"
         out[key] = {"concurrent_ms_per_round": round(tc * 1e3, 1), "sequential_ms_per_round": round(ts * 1e3, 1),
-                    "props_per_round": [pc, ps], "speedup": round(ts / tc, 3)}
+                    "props_per_round": [pc, ps], "speedup": round(ts / tc, 3),
+                    "concurrent_autotuned_ms_per_round": round(tt * 1e3, 1), "autotuned_props_per_round": pt,
+                    "autotuned_schedules": tuned_scheds[with_string]}
         print(f"uneven: {key} done", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 

@@ -874,89 +874,107 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
     return hipSuccess;
 }
 
-// The contour loop (begin + N steps) for nb slots. Replayed from a hipGraph captured once
-// per batch size when graphs are enabled (SCFT_GRAPH=0 launches every kernel directly).
+// The contour loop (begin + N steps) for nb slots. With graphs enabled (SCFT_GRAPH=1) each
+// replica group's loop is captured once per batch size and replayed on its own stream.
 template <typename T>
 hipError_t PropEngine<T>::contour(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st)
 {
-    if (!use_graph_) return contour_launch(a, nb, cc, st, fork_, join_);
-    for (auto& g : graphs_)
-        if (g.nb == nb && g.st == st) return hipGraphLaunch(g.exec, st);
-    GraphEntry ge{nb, st, nullptr, {}};
+    if (!use_graph_) return contour_launch(a, nb, cc, st);
+    PropArgs<T> g[kMaxStreams];
+    hipStream_t gs[kMaxStreams];
+    const int ng = groups(a, nb, st, g, gs);
+    const GraphEntry* ge = nullptr;
+    for (const auto& x : graphs_)
+        if (x.nb == nb && x.st == st) ge = &x;
     hipError_t e = hipSuccess;
-    for (int i = 0; i < kMaxStreams && e == hipSuccess; ++i)
-        e = hipEventCreateWithFlags(&ge.ev[i], hipEventDisableTiming);
-    hipGraph_t graph = nullptr;
-    if (e == hipSuccess) e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) {
-        hipError_t el = contour_launch(a, nb, cc, st, ge.ev[0], ge.ev + 1);
-        e = hipStreamEndCapture(st, &graph);
-        if (el != hipSuccess) e = el;
+    if (!ge) {
+        GraphEntry n{nb, st, ng, {}};
+        for (int i = 0; i < ng && e == hipSuccess; ++i) {
+            hipGraph_t graph = nullptr;
+            e = hipStreamBeginCapture(gs[i], hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) break;
+            begin_(g[i], gs[i]);
+            for (int s = 0; s < cc.N; ++s) step_(g[i], s, gs[i]);
+            const hipError_t el = hipGetLastError();
+            e = hipStreamEndCapture(gs[i], &graph);
+            if (e == hipSuccess) e = el;
+            if (e == hipSuccess) e = hipGraphInstantiate(&n.exec[i], graph, nullptr, nullptr, 0);
+            if (graph) (void)hipGraphDestroy(graph);
+        }
+        if (e != hipSuccess) {
+            for (hipGraphExec_t x : n.exec)
+                if (x) (void)hipGraphExecDestroy(x);
+            return e;
+        }
+        graphs_.push_back(n);
+        ge = &graphs_.back();
     }
-    if (e == hipSuccess) e = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
-    if (graph) (void)hipGraphDestroy(graph);
-    if (e == hipSuccess) e = hipGraphLaunch(ge.exec, st);
-    if (e != hipSuccess && !ge.exec) {
-        for (hipEvent_t ev : ge.ev)
-            if (ev) (void)hipEventDestroy(ev);
-        return e;
+    if (ng > 1) {
+        e = hipEventRecord(fork_, st);
+        for (int i = 1; i < ng && e == hipSuccess; ++i) e = hipStreamWaitEvent(gs[i], fork_, 0);
     }
-    graphs_.push_back(ge);
+    for (int i = 0; i < ng && e == hipSuccess; ++i) e = hipGraphLaunch(ge->exec[i], gs[i]);
+    for (int i = 1; i < ng && e == hipSuccess; ++i) {
+        e = hipEventRecord(join_[i - 1], gs[i]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, join_[i - 1], 0);
+    }
     return e;
 }
 
-// Destroys every captured contour loop with its events (after the work that used them is
-// complete: the callers synchronise first).
+// Destroys every captured contour loop (after the work that used them is complete: the
+// callers synchronise first).
 template <typename T>
 void PropEngine<T>::clear_graphs()
 {
-    for (auto& g : graphs_) {
-        (void)hipGraphExecDestroy(g.exec);
-        for (hipEvent_t ev : g.ev)
-            if (ev) (void)hipEventDestroy(ev);
-    }
+    for (auto& g : graphs_)
+        for (hipGraphExec_t x : g.exec)
+            if (x) (void)hipGraphExecDestroy(x);
     graphs_.clear();
 }
 
 template <typename T>
-hipError_t PropEngine<T>::contour_launch(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st, hipEvent_t fork,
-                                         const hipEvent_t* join)
+int PropEngine<T>::groups(const PropArgs<T>& a, int nb, hipStream_t st, PropArgs<T>* g, hipStream_t* gs) const
 {
-    const int ng = std::min(nstreams_, nb);
-    if (ng > 1) {
-        // Independent replica groups on their own streams: one group's LDS/latency-bound
-        // plane kernels co-run with another group's bandwidth-bound column kernels.
-        PropArgs<T> g[kMaxStreams];
-        hipStream_t gs[kMaxStreams];
-        int first = 0;
-        for (int i = 0; i < ng; ++i) {
-            const int cnt = nb / ng + (i < nb % ng ? 1 : 0);
-            const size_t off = (size_t)first * (size_t)M_;
-            g[i] = a;
-            g[i].nb = cnt;
-            g[i].Ch = Ch_ + off;
-            g[i].Cf = Cf_ + off;
-            g[i].e4A = e4A_ + off;
-            g[i].e4B = e4B_ + off;
-            g[i].qs = qs_ + off * (size_t)qpitch_;
-            gs[i] = i == 0 ? st : sx_[i - 1];
-            first += cnt;
-        }
-        hipError_t e = hipEventRecord(fork, st);
-        for (int i = 1; i < ng && e == hipSuccess; ++i) e = hipStreamWaitEvent(gs[i], fork, 0);
-        if (e != hipSuccess) return e;
-        for (int i = 0; i < ng; ++i) begin_(g[i], gs[i]);
-        for (int s = 0; s < cc.N; ++s)
-            for (int i = 0; i < ng; ++i) step_(g[i], s, gs[i]);
-        for (int i = 1; i < ng && e == hipSuccess; ++i) {
-            e = hipEventRecord(join[i - 1], gs[i]);
-            if (e == hipSuccess) e = hipStreamWaitEvent(st, join[i - 1], 0);
-        }
-        if (e != hipSuccess) return e;
-    } else {
-        begin_(a, st);
-        for (int s = 0; s < cc.N; ++s) step_(a, s, st);
+    const int ng = std::max(1, std::min(nstreams_, nb));
+    int first = 0;
+    for (int i = 0; i < ng; ++i) {
+        const int cnt = nb / ng + (i < nb % ng ? 1 : 0);
+        const size_t off = (size_t)first * (size_t)M_;
+        g[i] = a;
+        g[i].nb = cnt;
+        g[i].Ch = Ch_ + off;
+        g[i].Cf = Cf_ + off;
+        g[i].e4A = e4A_ + off;
+        g[i].e4B = e4B_ + off;
+        g[i].qs = qs_ + off * (size_t)qpitch_;
+        gs[i] = i == 0 ? st : sx_[i - 1];
+        first += cnt;
     }
+    return ng;
+}
+
+// Independent replica groups on their own streams: one group's LDS/latency-bound plane
+// kernels co-run with another group's bandwidth-bound column kernels.
+template <typename T>
+hipError_t PropEngine<T>::contour_launch(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st)
+{
+    PropArgs<T> g[kMaxStreams];
+    hipStream_t gs[kMaxStreams];
+    const int ng = groups(a, nb, st, g, gs);
+    hipError_t e = hipSuccess;
+    if (ng > 1) {
+        e = hipEventRecord(fork_, st);
+        for (int i = 1; i < ng && e == hipSuccess; ++i) e = hipStreamWaitEvent(gs[i], fork_, 0);
+        if (e != hipSuccess) return e;
+    }
+    for (int i = 0; i < ng; ++i) begin_(g[i], gs[i]);
+    for (int s = 0; s < cc.N; ++s)
+        for (int i = 0; i < ng; ++i) step_(g[i], s, gs[i]);
+    for (int i = 1; i < ng && e == hipSuccess; ++i) {
+        e = hipEventRecord(join_[i - 1], gs[i]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, join_[i - 1], 0);
+    }
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 

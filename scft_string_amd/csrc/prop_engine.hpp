@@ -103,10 +103,9 @@ private:
     PropArgs<T> args(int nb, const ChainConst& cc) const;
     hipError_t ensure_streams(int n);
     hipError_t contour(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st);
-    // fork / join: the events that order the replica-group streams after st and st after
-    // them (the engine's own for direct launches, the graph entry's own under capture)
-    hipError_t contour_launch(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st, hipEvent_t fork,
-                              const hipEvent_t* join);
+    // direct launches of the contour loop, the replica-group streams ordered after st and st
+    // after them with the engine's fork / join events
+    hipError_t contour_launch(PropArgs<T> a, int nb, const ChainConst& cc, hipStream_t st);
     // checkpointed store: recompute contour indices lo..hi into window w (entries
     // ck_window(w) + s - lo), from the checkpoint at or below lo
     void window(PropArgs<T> a, int w, int lo, int hi, hipStream_t st);
@@ -114,15 +113,18 @@ private:
     int store_index(int s, const ChainConst& cc) const { return ck_ ? ck_index(s, cc.N, cc.Nh, ck_) : s; }
     hipError_t windowed_density(const PropArgs<T>& a, int nb, const ChainConst& cc, double* dphi,
                                 long long phistride, const int* d_slot2rep, hipStream_t st);
-    // A captured contour loop. Its fork / join events are created for that capture and
-    // used by nothing else: an event recorded inside a capture refers to that capture's
-    // graph nodes, so one recorded in an earlier capture (whose graph is gone) must never
-    // order a later capture or a direct launch.
+    // The replica groups of an nb-slot batch: group i's arguments and stream (group 0 runs
+    // on st). Returns the group count.
+    int groups(const PropArgs<T>& a, int nb, hipStream_t st, PropArgs<T>* g, hipStream_t* gs) const;
+    // The captured contour loops of one batch size: one graph per replica group, each
+    // captured alone on its group's stream, so no capture holds an event or a second stream.
+    // Replay orders the group streams after st and st after them with the engine's own
+    // fork / join events, outside any capture.
     struct GraphEntry {
         int nb;
         hipStream_t st;
-        hipGraphExec_t exec;
-        hipEvent_t ev[kMaxStreams];  // [0] fork, [1..] join
+        int ng;
+        hipGraphExec_t exec[kMaxStreams];
     };
     std::vector<GraphEntry> graphs_;
     void clear_graphs();
