@@ -50,12 +50,14 @@ typedef enum {
     SCFT_SCHED_DEFAULT = 0,
     SCFT_SCHED_TWO_STAGE = 1, /* plane/column kernels on register FFTs (grids compiled for them) */
     SCFT_SCHED_STOCKHAM = 2,  /* plane/column kernels on LDS Stockham FFTs */
-    SCFT_SCHED_PENCIL = 3     /* one line pass per axis, any grid whose axes factor into 2, 3, 5 */
+    SCFT_SCHED_PENCIL = 3,    /* one line pass per axis, any grid whose axes factor into 2, 3, 5 */
+    SCFT_SCHED_TWO_PASS = 4   /* one plane pass and one column pass per contour step, the kinetic
+                                 factor split by axis (grids with the two-stage kernels) */
 } scft_sched_family;
 
 typedef struct scft_schedule {
     int family;   /* scft_sched_family */
-    int coltile;  /* two-stage: (y, z) columns per workgroup of the x pass (32 or 64) */
+    int coltile;  /* two-stage, two-pass: (y, z) columns per workgroup of the x pass (32 or 64) */
     int plane_nt; /* two-stage: threads per workgroup of the plane kernels (the grid's default
                      or half of it, for 512- and 768-thread planes) */
     int streams;  /* replica groups on their own streams, 1 .. 4 */

@@ -3,6 +3,7 @@
 #include "prop_kernels.hpp"
 #include "prop_kernels2.hpp"
 #include "prop_pencil.hpp"
+#include "prop_sandwich.hpp"
 #include "grad_kernels.hpp"
 
 #include <algorithm>
@@ -50,6 +51,13 @@ struct KS {
     // many threads, each with twice the items
     static constexpr int NT_ALT = (PG::NT >= 512 && PG::NT < 1024) ? PG::NT / 2 : 0;
     static constexpr bool palt_ok = two_ok && NT_ALT > 0;
+    // the two-pass step (prop_sandwich.hpp): its plane pass holds the (y, z) kinetic tables
+    // in LDS beside the two-stage plane's buffers
+    static constexpr size_t lds3_plane = lds2_plane + (size_t)(N1 + N2) * sizeof(T);
+    static constexpr size_t lds3_col = scol_lds_bytes<T, N0, TC>();
+    static constexpr size_t lds3_col_alt = scol_lds_bytes<T, N0, TC_ALT>();
+    static constexpr bool tp_ok = two_ok && lds3_plane <= 160 * 1024 && lds3_col <= 160 * 1024;
+    static constexpr bool tp_alt_ok = tp_ok && alt_ok && lds3_col_alt <= 160 * 1024;
     static_assert(legacy_ok || two_ok, "no kernel fits in LDS for this grid");
     static_assert(fft_len_supported(N0) && fft_len_supported(N1) && fft_len_supported(N2), "radix");
 
@@ -186,6 +194,49 @@ struct KS {
     {
         if constexpr (two_ok) plane_end_two(a, c, (int)PL_RESUME, st);
     }
+    static hipError_t setup_tp()
+    {
+        if constexpr (tp_ok) {
+            hipError_t e = hipFuncSetAttribute((const void*)k_splane2<T, N1, N2>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds3_plane);
+            if (e != hipSuccess) return e;
+            if constexpr (tp_alt_ok) {
+                e = hipFuncSetAttribute((const void*)k_scol2<T, N0, N1, N2, TC_ALT>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds3_col_alt);
+                if (e != hipSuccess) return e;
+            }
+            return hipFuncSetAttribute((const void*)k_scol2<T, N0, N1, N2, TC>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds3_col);
+        }
+        return hipErrorNotSupported;
+    }
+    // column pass of the two-pass step at the tile width a.coltile asks for (0: TC)
+    static void scol_tp(const PropArgs<T>& a, int s, int mode, hipStream_t st)
+    {
+        if constexpr (tp_alt_ok) {
+            if (a.coltile == TC_ALT) {
+                hipLaunchKernelGGL((k_scol2<T, N0, N1, N2, TC_ALT>), dim3(ntile_alt, a.nb), dim3(CGA::NT), lds3_col_alt,
+                                   st, a, s, mode);
+                return;
+            }
+        }
+        hipLaunchKernelGGL((k_scol2<T, N0, N1, N2, TC>), dim3(ntile, a.nb), dim3(CG::NT), lds3_col, st, a, s, mode);
+    }
+    static void begin_tp(const PropArgs<T>& a, hipStream_t st)
+    {
+        if constexpr (tp_ok) scol_tp(a, 0, (int)PL_BEGIN, st);
+    }
+    static void step_tp(const PropArgs<T>& a, int s, hipStream_t st)
+    {
+        if constexpr (tp_ok) {
+            hipLaunchKernelGGL((k_splane2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds3_plane, st, a, s);
+            scol_tp(a, s, (int)(s == a.N - 1 ? PL_LAST : PL_STEP), st);
+        }
+    }
+    static void resume_tp(const PropArgs<T>& a, int c, hipStream_t st)
+    {
+        if constexpr (tp_ok) scol_tp(a, c, (int)PL_RESUME, st);
+    }
     static hipError_t gsetup_two()
     {
         if constexpr (two_ok) {
@@ -223,6 +274,13 @@ struct KS {
                        lds_plane > 80 * 1024, TC, alt_ok ? TC_ALT : 0, persist_ok, nullptr, nullptr};
         k.plane_nt = PG::NT;
         k.plane_nt_alt = palt_ok ? NT_ALT : 0;
+        k.tc_alt3 = tp_alt_ok ? TC_ALT : 0;
+        if (tp_ok) {
+            k.setup3 = &setup_tp;
+            k.begin3 = &begin_tp;
+            k.step3 = &step_tp;
+            k.resume3 = &resume_tp;
+        }
         if (two_ok) {
             k.setup2 = &setup_two;
             k.begin2 = &begin_two;
@@ -468,48 +526,64 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
             pgrid_ = cus;
         if (const char* pg = getenv("SCFT_PLANE_GRID")) pgrid_ = std::max(0, atoi(pg));
     }
-    // Kernel family. Default: plane/column kernels where this grid has them (two-stage when
-    // it has that set), the line kernels otherwise or when SCFT_PENCIL=1; SCFT_LEGACY_FFT=1
-    // keeps the Stockham plane/column kernels. Without a two-stage set, a Stockham plane that
-    // fills a CU's LDS runs one workgroup per CU, and the line kernels are faster there (96^3
-    // FP64: 0.529 against 0.805 ms per contour step for 8 replicas).
+    // Kernel family. Default: plane/column kernels where this grid has them (the two-pass
+    // step when it has that set: 16-22% faster per contour step than the four-launch
+    // two-stage step on every grid measured), the line kernels otherwise or when
+    // SCFT_PENCIL=1; SCFT_LEGACY_FFT=1 keeps the Stockham plane/column kernels;
+    // SCFT_FAMILY=two-stage|two-pass|stockham|pencil names one. Without a two-stage set, a
+    // Stockham plane that fills a CU's LDS runs one workgroup per CU, and the line kernels
+    // are faster there (96^3 FP64: 0.529 against 0.805 ms per contour step for 8 replicas).
     int family = sc.family;
+    if (family == SCFT_SCHED_DEFAULT) {
+        const char* fe = getenv("SCFT_FAMILY");
+        if (fe && *fe) {
+            const std::string f(fe);
+            family = f == "two-stage" ? SCFT_SCHED_TWO_STAGE
+                     : f == "two-pass" ? SCFT_SCHED_TWO_PASS
+                     : f == "stockham" ? SCFT_SCHED_STOCKHAM
+                     : f == "pencil"   ? SCFT_SCHED_PENCIL
+                                       : SCFT_SCHED_DEFAULT;
+        }
+    }
     if (family == SCFT_SCHED_DEFAULT) {
         const char* pen = getenv("SCFT_PENCIL");
         const char* legacy = getenv("SCFT_LEGACY_FFT");
         const bool force_legacy = legacy && legacy[0] == '1';
         if ((pen && pen[0] == '1') || !kset) family = SCFT_SCHED_PENCIL;
+        else if (kset->step3 && !force_legacy) family = SCFT_SCHED_TWO_PASS;
         else if (kset->step2 && !force_legacy) family = SCFT_SCHED_TWO_STAGE;
         else if (!kset->step2 && kset->big_plane && !force_legacy && pencil_ok) family = SCFT_SCHED_PENCIL;
         else family = SCFT_SCHED_STOCKHAM;
     } else {
         // an explicit family must exist as such for this grid
         const bool have = family == SCFT_SCHED_TWO_STAGE ? kset && kset->step2
+                          : family == SCFT_SCHED_TWO_PASS ? kset && kset->step3
                           : family == SCFT_SCHED_STOCKHAM ? kset && kset->step != kset->step2
                           : family == SCFT_SCHED_PENCIL   ? pencil_ok
                                                           : false;
         if (!have) return "schedule: kernel family " + std::to_string(family) + " is not available for this grid";
     }
     hipError_t e = hipSuccess;
-    if (family == SCFT_SCHED_TWO_STAGE || family == SCFT_SCHED_STOCKHAM) {
+    if (family == SCFT_SCHED_TWO_STAGE || family == SCFT_SCHED_TWO_PASS || family == SCFT_SCHED_STOCKHAM) {
         ks_ = kset;
-        const bool use_two = family == SCFT_SCHED_TWO_STAGE;
-        setup_ = use_two ? ks_->setup2 : ks_->setup;
-        begin_ = use_two ? ks_->begin2 : ks_->begin;
-        step_ = use_two ? ks_->step2 : ks_->step;
-        resume_ = use_two ? ks_->resume2 : ks_->resume;
-        path_ = use_two ? "two-stage" : "stockham";
+        const bool use_two = family == SCFT_SCHED_TWO_STAGE, use_tp = family == SCFT_SCHED_TWO_PASS;
+        setup_ = use_tp ? ks_->setup3 : use_two ? ks_->setup2 : ks_->setup;
+        begin_ = use_tp ? ks_->begin3 : use_two ? ks_->begin2 : ks_->begin;
+        step_ = use_tp ? ks_->step3 : use_two ? ks_->step2 : ks_->step;
+        resume_ = use_tp ? ks_->resume3 : use_two ? ks_->resume2 : ks_->resume;
+        path_ = use_tp ? "two-pass" : use_two ? "two-stage" : "stockham";
         // column tile: the schedule's, else SCFT_COLTILE=<width>, else the grid's default
         int ct = sc.coltile;
-        if (ct == 0 && use_two) {
+        if (ct == 0 && (use_two || use_tp)) {
             const char* cte = getenv("SCFT_COLTILE");
-            if (cte && ks_->tc_alt && atoi(cte) == ks_->tc_alt) ct = ks_->tc_alt;
+            const int alt = use_tp ? ks_->tc_alt3 : ks_->tc_alt;
+            if (cte && alt && atoi(cte) == alt) ct = alt;
         }
-        if (ct != 0 && ct != ks_->tc && !(use_two && ct == ks_->tc_alt))
+        if (ct != 0 && ct != ks_->tc && !((use_two && ct == ks_->tc_alt) || (use_tp && ct == ks_->tc_alt3)))
             return "schedule: column tile " + std::to_string(ct) + " is not available for this grid";
         if (ct != 0 && ct != ks_->tc) {
             coltile_ = ct;
-            path_ = ct == 32 ? "two-stage-c32" : "two-stage-c64";
+            path_ = use_tp ? (ct == 32 ? "two-pass-c32" : "two-pass-c64") : (ct == 32 ? "two-stage-c32" : "two-stage-c64");
         }
         // plane workgroup size: the schedule's, else SCFT_PLANE_NT=<threads>, else the default
         int pn = sc.plane_nt;
@@ -517,7 +591,7 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
             const char* pne = getenv("SCFT_PLANE_NT");
             if (pne && ks_->plane_nt_alt && atoi(pne) == ks_->plane_nt_alt) pn = ks_->plane_nt_alt;
         }
-        if (pn != 0 && !(use_two && (pn == ks_->plane_nt || pn == ks_->plane_nt_alt)))
+        if (pn != 0 && !(use_two && (pn == ks_->plane_nt || pn == ks_->plane_nt_alt)) && !(use_tp && pn == ks_->plane_nt))
             return "schedule: plane workgroup size " + std::to_string(pn) + " is not available for this grid";
         if (pn != 0 && pn != ks_->plane_nt) planent_ = pn;
         e = setup_();
@@ -594,6 +668,11 @@ std::vector<scft_schedule> PropEngine<T>::candidates(const int m[3], int nb)
         for (int ct : cts)
             for (int pn : pns)
                 if (ct && pn) fam.push_back({SCFT_SCHED_TWO_STAGE, ct, pn, 0});
+    }
+    if (k && k->step3) {
+        const int cts[2] = {k->tc, k->tc_alt3};
+        for (int ct : cts)
+            if (ct) fam.push_back({SCFT_SCHED_TWO_PASS, ct, k->plane_nt, 0});
     }
     if (k && k->step != k->step2) fam.push_back({SCFT_SCHED_STOCKHAM, 0, 0, 0});
     if (pencil_supported(m[0], m[1], m[2])) fam.push_back({SCFT_SCHED_PENCIL, 0, 0, 0});

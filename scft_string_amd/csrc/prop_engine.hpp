@@ -39,6 +39,13 @@ struct KernelSet {
                    cpx<T>* Kf, hipStream_t);
     // two-stage plane kernels: the default workgroup size and the alternative (0: none)
     int plane_nt, plane_nt_alt;
+    // the two-pass step (prop_sandwich.hpp; nullptr when the grid has none) and its other
+    // column tile width (0: none)
+    int tc_alt3;
+    hipError_t (*setup3)();
+    void (*begin3)(const PropArgs<T>&, hipStream_t);
+    void (*step3)(const PropArgs<T>&, int s, hipStream_t);
+    void (*resume3)(const PropArgs<T>&, int c, hipStream_t);
 };
 
 // kernel set for an (m0, m1, m2) grid, or nullptr when no instantiation exists

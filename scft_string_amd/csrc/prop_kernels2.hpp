@@ -222,6 +222,100 @@ struct P2 : P2Geom<NY, NZ, NTO> {
         }
     }
 
+    // forward 2-D transform of the Z1-layout registers zr, the spectrum left in registers in
+    // Y2 ownership (u[it][l1] = Y[l2 + B2 l1][kz] of item it = (kz, l2)), the layout inv_regs
+    // starts from. The same stages as fwd (kept as a separate copy: sharing the stages through
+    // a common helper changed the register allocation of the FP64 64^2 plane_end, which then
+    // spilled). buf must be free on entry; it is in use on return.
+    static __device__ __forceinline__ void fwd_regs(cpx<T> (&zr)[IZ1][A2], cpx<T>* buf, const cpx<T>* twz,
+                                                    const cpx<T>* twy, cpx<T> (&uo)[IY2][B1], int tid)
+    {
+        // z stage 1
+#pragma unroll
+        for (int it = 0; it < IZ1; ++it) {
+            const int w = tid + it * NT;
+            if ((NY * A1) % NT == 0 || w < NY * A1) {
+                const int y = w / A1, a1 = w % A1;
+                rdft<T, A2, false>(zr[it]);
+                sfor<A2>([&](auto kc) {
+                    constexpr int k2 = decltype(kc)::value;
+                    const cpx<T> v = k2 == 0 ? zr[it][0] : cmul(zr[it][k2], twz[(a1 * k2) % NZ]);
+                    buf[y * RZ + a1 * PZ + k2] = v;
+                });
+            }
+        }
+        zsync();
+        // z stage 2
+        cpx<T> zz[IZ2E][A1];
+#pragma unroll
+        for (int it = 0; it < IZ2E; ++it) {
+            int y, k2;
+            if (z2_item(tid, it, y, k2)) {
+                sfor<A1>([&](auto ac) {
+                    constexpr int a1 = decltype(ac)::value;
+                    zz[it][a1] = buf[y * RZ + a1 * PZ + k2];
+                });
+                rdft<T, A1, false>(zz[it]);
+            }
+        }
+        zsync();
+#pragma unroll
+        for (int it = 0; it < IZ2E; ++it) {
+            int y, k2;
+            if (z2_item(tid, it, y, k2)) {
+                sfor<A1>([&](auto kc) {
+                    constexpr int k1 = decltype(kc)::value;
+                    buf[y * PP + k2 + A2 * k1] = zz[it][k1];
+                });
+            }
+        }
+        __syncthreads();
+        // y stage 1
+        cpx<T> yy[IY1][B2];
+#pragma unroll
+        for (int it = 0; it < IY1; ++it) {
+            const int w = tid + it * NT;
+            if ((NZ * B1) % NT == 0 || w < NZ * B1) {
+                const int kz = w % NZ, b1 = w / NZ;
+                sfor<B2>([&](auto bc) {
+                    constexpr int b2 = decltype(bc)::value;
+                    yy[it][b2] = buf[(b1 + B1 * b2) * PP + kz];
+                });
+                rdft<T, B2, false>(yy[it]);
+                sfor<B2>([&](auto lc) {
+                    constexpr int l2 = decltype(lc)::value;
+                    if (l2 != 0) yy[it][l2] = cmul(yy[it][l2], twy[(b1 * l2) % NY]);
+                });
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IY1; ++it) {
+            const int w = tid + it * NT;
+            if ((NZ * B1) % NT == 0 || w < NZ * B1) {
+                const int kz = w % NZ, b1 = w / NZ;
+                sfor<B2>([&](auto lc) {
+                    constexpr int l2 = decltype(lc)::value;
+                    buf[kz * CY + b1 * PY + l2] = yy[it][l2];
+                });
+            }
+        }
+        __syncthreads();
+        // y stage 2 -> registers
+#pragma unroll
+        for (int it = 0; it < IY2; ++it) {
+            const int w = tid + it * NT;
+            if ((NZ * B2) % NT == 0 || w < NZ * B2) {
+                const int kz = w % NZ, l2 = w / NZ;
+                sfor<B1>([&](auto bc) {
+                    constexpr int b1 = decltype(bc)::value;
+                    uo[it][b1] = buf[kz * CY + b1 * PY + l2];
+                });
+                rdft<T, B1, false>(uo[it]);
+            }
+        }
+    }
+
     // k-space input of the inverse, in Y2 ownership, read straight from global memory
     static __device__ __forceinline__ void inv_load(const cpx<T>* __restrict__ in, cpx<T> (&u)[IY2][B1], int tid)
     {

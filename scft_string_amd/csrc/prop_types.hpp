@@ -156,6 +156,8 @@ struct Kinetic;
 template <>
 struct Kinetic<double> {
     static double table(double a) { return exp(-a); }
+    // the factor of an axis left out (a = 0)
+    static __host__ __device__ __forceinline__ double one() { return 1.0; }
     // per-column part (y, z)
     static __host__ __device__ __forceinline__ double col(double gy, double gz) { return gy * gz; }
     static __host__ __device__ __forceinline__ void apply(cpx<double>& v, double gx, double gyz, bool full,
@@ -172,6 +174,7 @@ struct Kinetic<double> {
 template <>
 struct Kinetic<float> {
     static float table(double a) { return (float)(-expm1(-a)); }
+    static __host__ __device__ __forceinline__ float one() { return 0.0f; }
     static __host__ __device__ __forceinline__ float col(float ky, float kz) { return ky + kz - ky * kz; }
     static __host__ __device__ __forceinline__ void apply(cpx<float>& v, float kx, float kyz, bool full, float invM)
     {

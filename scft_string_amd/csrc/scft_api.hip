@@ -353,12 +353,14 @@ scft_status anderson_iterate(scft_ctx* c, double* alf, scft_iter_report* rep)
     return SCFT_OK;
 }
 
-// "two-stage c64 nt512 s2", "stockham s1", "pencil s3"
+// "two-stage c64 nt512 s2", "two-pass c64 s2", "stockham s1", "pencil s3"
 std::string sched_name(const scft_schedule& s)
 {
     char b[96];
     if (s.family == SCFT_SCHED_TWO_STAGE)
         snprintf(b, sizeof b, "two-stage c%d nt%d s%d", s.coltile, s.plane_nt, s.streams);
+    else if (s.family == SCFT_SCHED_TWO_PASS)
+        snprintf(b, sizeof b, "two-pass c%d s%d", s.coltile, s.streams);
     else
         snprintf(b, sizeof b, "%s s%d", s.family == SCFT_SCHED_STOCKHAM ? "stockham" : "pencil", s.streams);
     return b;
@@ -417,7 +419,7 @@ scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
     if (p->anderson < 0 || p->anderson > 40 || (p->anderson > 0 && p->dostring != 0)) return SCFT_ERR_INVALID;
     if (p->ckpt < 0) return SCFT_ERR_INVALID;
     const scft_schedule& sc = p->sched;
-    if (sc.family < SCFT_SCHED_DEFAULT || sc.family > SCFT_SCHED_PENCIL || sc.coltile < 0 || sc.plane_nt < 0 ||
+    if (sc.family < SCFT_SCHED_DEFAULT || sc.family > SCFT_SCHED_TWO_PASS || sc.coltile < 0 || sc.plane_nt < 0 ||
         sc.streams < 0 || sc.streams > 4)
         return SCFT_ERR_INVALID;
     // autotune = 1: the schedule comes from scft_autotune, which times every candidate in a

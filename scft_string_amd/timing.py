@@ -2,7 +2,7 @@
 
     python -m scft_string_amd.timing --m 200 150 150 --D 16 12 12 --N 60 --ens 2 \\
         --phidb 1.16e-5 --f 0.8 --chi 30 --prec fp64 --nrep 1 [--pencil] [--runtime-lengths]
-        [--fields lamella] [--copy-probe]
+        [--fields lamella] [--copy-probe] [--family two-pass] [--streams 3] [--coltile 32]
 
 prints one JSON line: ms per contour step (the mean over reps of a props of nrep replicas,
 divided by N), the kernel family, and the algorithmic HBM rate against the 8 TB/s peak.
@@ -11,10 +11,11 @@ divided by N), the kernel family, and the algorithmic HBM rate against the 8 TB/
 (SCFT_PENCIL_RT=1). --fields lamella times a lamellar field state instead of W = 0: the
 step time depends on the data, and zero fields run measurably faster.
 
-    python -m scft_string_amd.timing --stream-probe
+    python -m scft_string_amd.timing --stream-probe [MiB ...]
 
-prints the streaming rates of torch's own copy and add kernels (256 MiB to 4 GiB), a
-practical ceiling to set the contour step's measured HBM traffic against.
+prints the streaming rates of torch's own copy and add kernels (by default 256 MiB to 4 GiB;
+arrays of a few tens of MiB stay in the 256 MB last-level cache), a practical ceiling to set
+the contour step's measured traffic against.
 """
 import argparse
 import json
@@ -73,8 +74,11 @@ def stream_probe(sizes_mib=(256, 1024, 4096), iters=20):
 
 
 def main(argv=None):
-    if "--stream-probe" in (sys.argv[1:] if argv is None else argv):
-        stream_probe()
+    args = sys.argv[1:] if argv is None else argv
+    if "--stream-probe" in args:
+        # --stream-probe [MiB ...]: array sizes (default 256 1024 4096)
+        sizes = [int(v) for v in args[args.index("--stream-probe") + 1:] if v.isdigit()]
+        stream_probe(tuple(sizes) if sizes else (256, 1024, 4096))
         return
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--m", type=int, nargs=3, required=True)
@@ -91,6 +95,10 @@ def main(argv=None):
     ap.add_argument("--pencil", action="store_true")
     ap.add_argument("--runtime-lengths", action="store_true")
     ap.add_argument("--fields", choices=("zero", "lamella"), default="zero")
+    ap.add_argument("--family", choices=("default", "two-stage", "two-pass", "stockham", "pencil"), default="default",
+                    help="propagator schedule: kernel family (scft_schedule.family)")
+    ap.add_argument("--streams", type=int, default=0, help="schedule: replica-group streams (0: default)")
+    ap.add_argument("--coltile", type=int, default=0, help="schedule: column tile (0: default)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first a 512 MiB device copy: the known-size scale for FETCH_SIZE / WRITE_SIZE")
     a = ap.parse_args(argv)
@@ -108,15 +116,17 @@ def main(argv=None):
     os.environ["SCFT_PENCIL_RT"] = "1" if a.runtime_lengths else "0"
     from .capi import Context
 
+    sched = {"family": a.family, "streams": a.streams, "coltile": a.coltile}
     with Context(tuple(a.m), tuple(a.D), a.N, a.f, a.alpha, a.ens, a.phidb, a.chi, nrep=a.nrep,
-                 precision=a.prec) as ctx:
+                 precision=a.prec, schedule=sched) as ctx:
         if a.fields == "lamella":
             ctx.set_fields(lamella_fields(a.m, a.D, a.nrep))
         ms = ctx.time_props(a.nrep, a.reps)
         path = ctx.fft_path
+        engine = ctx.engine_info
     p = 4 if a.prec == "fp32" else 8
     gbs = a.nrep * algorithmic_bytes_per_step(a.m, p) / (ms * 1e-3) / 1e9
-    print(json.dumps({"m": a.m, "D": a.D, "N": a.N, "prec": a.prec, "nrep": a.nrep, "path": path,
+    print(json.dumps({"m": a.m, "D": a.D, "N": a.N, "prec": a.prec, "nrep": a.nrep, "path": path, "engine": engine,
                       "runtime_lengths": a.runtime_lengths, "fields": a.fields,
                       "ms_per_step": round(ms, 4), "alg_GBps": round(gbs, 1),
                       "frac": round(gbs / HBM_PEAK_GBS, 3)}), flush=True)

@@ -26,7 +26,8 @@ def test_help_lists_the_modes():
 
 def test_traffic_counts_the_contour_kernels_of_every_schedule():
     """measure_traffic sums the contour loop's kernels whatever the schedule (two-stage at
-    either tile or plane size, Stockham, line passes) and none of the props' other kernels."""
+    either tile or plane size, two-pass, Stockham, line passes) and none of the props' other
+    kernels."""
     sys.path.insert(0, ROOT)
     import bench
 
@@ -35,6 +36,8 @@ def test_traffic_counts_the_contour_kernels_of_every_schedule():
            "void scft::k_plane_mid2<float, 64, 64, 0>(scft::PropArgs<float>, int)",
            "void scft::k_plane_end2<float, 64, 64, 256>(scft::PropArgs<float>, int, int)",
            "void scft::k_plane_mid2p<float, 96, 96>(scft::PropArgs<float>, int, int)",
+           "void scft::k_splane2<float, 64, 64>(scft::PropArgs<float>, int)",
+           "void scft::k_scol2<float, 64, 64, 64, 32>(scft::PropArgs<float>, int, int)",
            "void scft::k_column<float, 64, 64, 64, 64, 256>(scft::PropArgs<float>, int, int)",
            "void scft::k_plane_end<float, 64, 64, 256>(scft::PropArgs<float>, int, int)",
            "void scft::k_pz<float, 64>(scft::PropArgs<float>, int, int, int)",

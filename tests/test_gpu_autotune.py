@@ -32,16 +32,17 @@ def _fields(m, D, R, seed=2):
 
 
 def _cands(report):
-    return re.findall(r"([a-z-]+(?: c\d+ nt\d+)? s\d) ([0-9.]+);", report)
+    return re.findall(r"([a-z-]+(?: c\d+)?(?: nt\d+)? s\d) ([0-9.]+);", report)
 
 
-@pytest.mark.parametrize("m,R,prec,nfam", [((64, 64, 64), 3, "fp32", 6), ((32, 32, 32), 3, "fp64", 4),
+@pytest.mark.parametrize("m,R,prec,nfam", [((64, 64, 64), 3, "fp32", 8), ((32, 32, 32), 3, "fp64", 6),
                                            ((16, 16, 16), 2, "fp64", 2)])
 def test_autotune_reports_every_candidate_and_picks_the_fastest(m, R, prec, nfam):
     """64^3 has the two-stage kernels at two column tiles x two plane workgroup sizes, the
-    Stockham kernels and the line kernels (6 kernel choices); 32^3 the two-stage kernels at
-    two column tiles and one plane size, Stockham, line (4); 16^3, whose axes are a single
-    register DFT and so have no two-stage set, Stockham and line (2)."""
+    two-pass step at two column tiles, the Stockham kernels and the line kernels (8 kernel
+    choices); 32^3 the two-stage kernels at two column tiles and one plane size, the two-pass
+    step at two column tiles, Stockham, line (6); 16^3, whose axes are a single register DFT
+    and so have no two-stage set, Stockham and line (2)."""
     _say("autotune candidates", m, prec, R)
     D = (4.0, 4.0, 4.0)
     best, rep = autotune(m, D, 30, 0.45, 0.1, 1, 0.8, 15.0, nrep=R, precision=prec, dostring=0,
@@ -78,6 +79,7 @@ def test_autotuned_context_matches_default(m, R):
 
 @pytest.mark.parametrize("sched", [
     {"family": "two-stage", "coltile": 32, "plane_nt": 256, "streams": 3},
+    {"family": "two-pass", "coltile": 32, "streams": 2},
     {"family": "two-stage", "coltile": 64, "plane_nt": 512, "streams": 1},
     {"family": "stockham", "streams": 2},
     {"family": "pencil", "streams": 3},

@@ -208,3 +208,29 @@ def test_saddles_converged_independently_in_fp32_and_fp64():
     assert np.ptp(fe["fp64"]) < 1e-9                      # three symmetric saddles
     assert np.abs(d).max() <= 1e-6 and np.abs(d - d[0]).max() <= 1e-6
     assert np.abs(d_all).max() <= 1e-6 and np.abs(d_all - d_all[0]).max() <= 1e-6
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_fp32_props_of_a_lamella_along_each_axis_within_1e6(axis):
+    """A lamella along one axis makes every line (x) or every plane (y, z) of a pass carry the
+    same data, so any rounding of the mean that a pass makes is the same everywhere: the
+    coherent case. One all-FP32 props at the benchmark's grid and contour length (64^3,
+    N = 128) against FP64: |dF| <= 1e-6. lnQ carries more of the FP32 error than F does
+    (measured |d lnQ| 1.5e-6 along x with the two-pass step, 1.7e-6 with the four-launch
+    step, against |dF| 2.6e-7 and 3.4e-7): its bar is 3e-6."""
+    m, D, N = (64, 64, 64), (6.0, 6.0, 6.0), 128
+    M = int(np.prod(m))
+    g = np.meshgrid(*[2 * np.pi * np.arange(m[i]) / m[i] for i in range(3)], indexing="ij")
+    W = np.zeros((1, 2, M))
+    W[0, 0] = (2.0 * np.cos(g[axis])).reshape(-1)
+    res = {}
+    for prec in ("fp32", "fp64"):
+        with Context(m, D, N, 0.45, 0.1, 1, 0.9, 18.0, nrep=1, precision=prec) as ctx:
+            phi, lnQ, _ = ctx.props_batch(W)
+        wa, wb = W[0, 0] + W[0, 1], W[0, 1] - W[0, 0]
+        F = -lnQ[0] + np.mean(18.0 * phi[0, 0] * phi[0, 1] - wa * phi[0, 0] - wb * phi[0, 1])
+        res[prec] = (lnQ[0], F)
+    dF = abs(res["fp32"][1] - res["fp64"][1])
+    dl = abs(res["fp32"][0] - res["fp64"][0])
+    _say(f"lamella along axis {axis}: all-FP32 |dF| {dF:.2e}, |dlnQ| {dl:.2e} (lnQ {res['fp64'][0]:.4f})")
+    assert dF <= 1e-6 and dl <= 3e-6

@@ -77,6 +77,7 @@ def test_props_fp32_error_vs_oracle(m):
     assert dphi < 1e-4
 
 
+@pytest.mark.parametrize("fam", ["two-stage", "two-pass"])
 @pytest.mark.parametrize("m,prec,tol", [
     ((32, 32, 32), "fp64", 1e-12), ((32, 32, 32), "fp32", 2e-5),
     ((48, 48, 48), "fp64", 1e-12),      # z split 4 x 12 (wave-local z exchanges)
@@ -84,18 +85,19 @@ def test_props_fp32_error_vs_oracle(m):
     ((64, 64, 64), "fp32", 2e-5),
     ((128, 128, 128), "fp32", 2e-5),    # z split 8 x 16, two stage-2 items per lane
 ], ids=lambda v: "x".join(map(str, v)) if isinstance(v, tuple) else str(v))
-def test_symmetric_operator_invariant(m, prec, tol):
+def test_symmetric_operator_invariant(m, prec, tol, fam):
     """<q1[s] q2[N-s]> is independent of s (fails for a wrong species switch, factor
-    order or Richardson weight), on each plane-kernel geometry."""
+    order or Richardson weight), on each plane-kernel geometry, for the four-launch
+    two-stage step and the two-pass step."""
     D = tuple(v / 8.0 for v in m)
     N = 48
     W = _fields(m, D, seed=5, amp=3.0)[None]
-    with Context(m, D, N, 0.4, 0.1, 1, 0.8, chi=15.0, nrep=1, precision=prec) as ctx:
-        assert ctx.fft_path == "two-stage"
+    with Context(m, D, N, 0.4, 0.1, 1, 0.8, chi=15.0, nrep=1, precision=prec, schedule={"family": fam}) as ctx:
+        assert ctx.fft_path == fam
         ctx.props_batch(W)
         qq = ctx.qq_invariant(0)
     spread = (qq.max() - qq.min()) / qq.mean()
-    _say(f"{m} {prec}: qq spread {spread:.3e}")
+    _say(f"{m} {prec} {fam}: qq spread {spread:.3e}")
     assert spread < tol
 
 
@@ -213,7 +215,8 @@ def test_persistent_plane_mid_matches_per_plane(pgrid, monkeypatch):
     out = {}
     for pg in ("0", pgrid):
         monkeypatch.setenv("SCFT_PLANE_GRID", pg)
-        with Context(m, D, 24, 0.45, 0.1, 1, 0.9, chi=18.0, nrep=3, precision="fp32") as ctx:
+        with Context(m, D, 24, 0.45, 0.1, 1, 0.9, chi=18.0, nrep=3, precision="fp32",
+                     schedule={"family": "two-stage"}) as ctx:
             assert ctx.fft_path == "two-stage"
             assert ("persistent plane_mid" in ctx.engine_info) == (pg != "0"), ctx.engine_info
             phi, lnQ, _ = ctx.props_batch(W)
@@ -259,18 +262,20 @@ def test_two_stage_and_stockham_paths_agree(prec, monkeypatch):
     assert abs(out["0"][1][0] - out["1"][1][0]) <= tol * max(1.0, abs(out["1"][1][0]))
 
 
+@pytest.mark.parametrize("fam", ["two-stage", "two-pass"])
 @pytest.mark.parametrize("m,prec,width", [((64, 64, 64), "fp32", 32), ((48, 48, 48), "fp64", 64),
                                           ((32, 32, 32), "fp64", 32)])
-def test_column_tile_widths_agree(m, prec, width, monkeypatch):
-    """The two-stage column pass at either tile width (the autotuner may pick either)
-    transforms every column with the same arithmetic: props are bit-identical."""
+def test_column_tile_widths_agree(m, prec, width, fam, monkeypatch):
+    """The column pass at either tile width (the autotuner may pick either) transforms every
+    column with the same arithmetic: props are bit-identical, for the two-stage and the
+    two-pass step."""
     D = (5.0, 5.0, 5.0)
     W = np.stack([_fields(m, D, seed=30 + i) for i in range(2)])
     out = {}
     for ct in ("0", str(width)):
         monkeypatch.setenv("SCFT_COLTILE", ct)
-        with Context(m, D, 24, 0.45, 0.1, 1, 0.9, chi=15.0, nrep=2, precision=prec) as ctx:
-            assert ctx.fft_path == ("two-stage" if ct == "0" else f"two-stage-c{width}")
+        with Context(m, D, 24, 0.45, 0.1, 1, 0.9, chi=15.0, nrep=2, precision=prec, schedule={"family": fam}) as ctx:
+            assert ctx.fft_path == (fam if ct == "0" else f"{fam}-c{width}")
             out[ct] = ctx.props_batch(W)
     assert np.array_equal(out["0"][0], out[str(width)][0]) and np.array_equal(out["0"][1], out[str(width)][1])
 
@@ -287,7 +292,8 @@ def test_plane_workgroup_sizes_agree(m, prec, nt, monkeypatch):
     out = {}
     for pn in ("0", str(nt)):
         monkeypatch.setenv("SCFT_PLANE_NT", pn)
-        with Context(m, D, 24, 0.45, 0.1, 1, 0.9, chi=18.0, nrep=2, precision=prec) as ctx:
+        with Context(m, D, 24, 0.45, 0.1, 1, 0.9, chi=18.0, nrep=2, precision=prec,
+                     schedule={"family": "two-stage"}) as ctx:
             assert ("%d-thread plane workgroups" % nt in ctx.engine_info) == (pn != "0"), ctx.engine_info
             phi, lnQ, _ = ctx.props_batch(W)
             qq = ctx.qq_invariant(1)

@@ -111,20 +111,20 @@ def test_pencil_symmetric_operator_invariant(prec, tol):
 
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
-def test_pencil_and_two_stage_agree(prec, tol, monkeypatch):
-    # 48 = 4 x 12 (z) and 6 x 8 (y) in the two-stage kernels, 8 x 2 x 3 in the line kernels. At 64 both run
-    # the same radix-8 four-step arithmetic, and FP32 agrees bit for bit.
+def test_pencil_and_plane_column_agree(prec, tol, monkeypatch):
+    # 48 = 4 x 12 (z) and 6 x 8 (y) in the plane/column kernels (the default two-pass step), 8 x 2 x 3 in
+    # the line kernels.
     m, D = (48, 48, 48), (5.0, 5.0, 5.0)
     W = np.stack([_fields(m, D, seed=40 + i) for i in range(3)])
     out = {}
     for pen in ("0", "1"):
         monkeypatch.setenv("SCFT_PENCIL", pen)
         with Context(m, D, 48, 0.45, 0.1, 1, 0.9, chi=18.0, nrep=3, precision=prec) as ctx:
-            assert ctx.fft_path == ("pencil" if pen == "1" else "two-stage")
+            assert ctx.fft_path == ("pencil" if pen == "1" else "two-pass")
             out[pen] = ctx.props_batch(W)
     dl = float(np.max(np.abs(out["0"][1] - out["1"][1])))
     dphi = _rel(out["1"][0], out["0"][0])
-    _say(f"pencil vs two-stage {prec}: |dlnQ| {dl:.2e}, rel dphi {dphi:.2e}")
+    _say(f"pencil vs two-pass {prec}: |dlnQ| {dl:.2e}, rel dphi {dphi:.2e}")
     assert dl <= tol and dphi <= 10 * tol
 
 
