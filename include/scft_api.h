@@ -51,8 +51,10 @@ typedef enum {
     SCFT_SCHED_TWO_STAGE = 1, /* plane/column kernels on register FFTs (grids compiled for them) */
     SCFT_SCHED_STOCKHAM = 2,  /* plane/column kernels on LDS Stockham FFTs */
     SCFT_SCHED_PENCIL = 3,    /* one line pass per axis, any grid whose axes factor into 2, 3, 5 */
-    SCFT_SCHED_TWO_PASS = 4   /* one plane pass and one column pass per contour step, the kinetic
+    SCFT_SCHED_TWO_PASS = 4,  /* one plane pass and one column pass per contour step, the kinetic
                                  factor split by axis (grids with the two-stage kernels) */
+    SCFT_SCHED_PENCIL_SPLIT = 5 /* line passes with the kinetic factor split by axis: x, y, z, y
+                                   per contour step, any grid the line kernels take */
 } scft_sched_family;
 
 typedef struct scft_schedule {

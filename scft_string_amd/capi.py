@@ -33,9 +33,10 @@ class ScftError(RuntimeError):
         self.code = code
 
 
-SCFT_SCHED_DEFAULT, SCFT_SCHED_TWO_STAGE, SCFT_SCHED_STOCKHAM, SCFT_SCHED_PENCIL, SCFT_SCHED_TWO_PASS = range(5)
+(SCFT_SCHED_DEFAULT, SCFT_SCHED_TWO_STAGE, SCFT_SCHED_STOCKHAM, SCFT_SCHED_PENCIL, SCFT_SCHED_TWO_PASS,
+ SCFT_SCHED_PENCIL_SPLIT) = range(6)
 _FAMILY = {"default": SCFT_SCHED_DEFAULT, "two-stage": SCFT_SCHED_TWO_STAGE, "stockham": SCFT_SCHED_STOCKHAM,
-           "pencil": SCFT_SCHED_PENCIL, "two-pass": SCFT_SCHED_TWO_PASS}
+           "pencil": SCFT_SCHED_PENCIL, "two-pass": SCFT_SCHED_TWO_PASS, "pencil-split": SCFT_SCHED_PENCIL_SPLIT}
 
 
 class Schedule(C.Structure):
@@ -50,7 +51,7 @@ class Schedule(C.Structure):
 
 def make_schedule(sched):
     """Schedule from None, a Schedule, or a dict with any of family ("two-stage", "two-pass",
-    "stockham", "pencil"), coltile, plane_nt, streams."""
+    "stockham", "pencil", "pencil-split"), coltile, plane_nt, streams."""
     if sched is None:
         return Schedule()
     if isinstance(sched, Schedule):

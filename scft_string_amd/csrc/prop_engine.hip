@@ -4,6 +4,7 @@
 #include "prop_kernels2.hpp"
 #include "prop_pencil.hpp"
 #include "prop_sandwich.hpp"
+#include "prop_lines2.hpp"
 #include "grad_kernels.hpp"
 
 #include <algorithm>
@@ -487,6 +488,96 @@ double PropEngine<T>::loop_ms() const
     return (double)ms;
 }
 
+// ---------------------------------------------------------------- split line step
+// (prop_lines2.hpp): k_ly (h, f), k_lz, k_ly (h), k_lx per contour step
+template <typename T>
+static hipError_t lines_setup()
+{
+    const int lds = (int)pencil_lds_max<T>();
+    std::vector<const void*> fns = {(const void*)k_ly<T, 0>, (const void*)k_lz<T, 0>, (const void*)k_lx<T, 0>};
+#define PC_FNS(L)                                                                                                 \
+    fns.push_back((const void*)k_ly<T, L>);                                                                       \
+    fns.push_back((const void*)k_lz<T, L>);                                                                       \
+    fns.push_back((const void*)k_lx<T, L>);
+    PC_CT_LENGTHS(PC_FNS)
+#undef PC_FNS
+    for (const void* f : fns) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+template <typename T>
+static void lines_ly(const PropArgs<T>& a, int nh, int nfields, const PencilTiles& p, size_t lds, hipStream_t st)
+{
+    const int ntz = (a.m2 + p.ty - 1) / p.ty;
+    const dim3 g(a.m0 * ntz, nfields);
+    if (a.pencil_ct) switch (a.m1) {
+#define PC_CASE(L) \
+    case L: hipLaunchKernelGGL((k_ly<T, L>), g, dim3(PC_NT), lds, st, a, nh, p.ty); return;
+            PC_CT_LENGTHS(PC_CASE)
+#undef PC_CASE
+            default: break;
+        }
+    hipLaunchKernelGGL((k_ly<T, 0>), g, dim3(PC_NT), lds, st, a, nh, p.ty);
+}
+
+template <typename T>
+static void lines_lz(const PropArgs<T>& a, int s, const PencilTiles& p, size_t lds, hipStream_t st)
+{
+    const long long lines = (long long)a.m0 * a.m1;
+    const dim3 g((unsigned)((lines + p.nlz - 1) / p.nlz), a.nb);
+    if (a.pencil_ct) switch (a.m2) {
+#define PC_CASE(L) \
+    case L: hipLaunchKernelGGL((k_lz<T, L>), g, dim3(PC_NT), lds, st, a, s, p.nlz); return;
+            PC_CT_LENGTHS(PC_CASE)
+#undef PC_CASE
+            default: break;
+        }
+    hipLaunchKernelGGL((k_lz<T, 0>), g, dim3(PC_NT), lds, st, a, s, p.nlz);
+}
+
+template <typename T>
+static void lines_lx(const PropArgs<T>& a, int s, int mode, const PencilTiles& p, size_t lds, hipStream_t st)
+{
+    const int ntz = (a.m2 + p.tx - 1) / p.tx;
+    const dim3 g(a.m1 * ntz, a.nb);
+    if (a.pencil_ct) switch (a.m0) {
+#define PC_CASE(L) \
+    case L: hipLaunchKernelGGL((k_lx<T, L>), g, dim3(PC_NT), lds, st, a, s, mode, p.tx); return;
+            PC_CT_LENGTHS(PC_CASE)
+#undef PC_CASE
+            default: break;
+        }
+    hipLaunchKernelGGL((k_lx<T, 0>), g, dim3(PC_NT), lds, st, a, s, mode, p.tx);
+}
+
+template <typename T>
+static void lines_begin(const PropArgs<T>& a, hipStream_t st)
+{
+    const PencilTiles p = pencil_tiles<T>(a.m0, a.m1, a.m2, a.pencil_ct);
+    lines_lx<T>(a, 0, PZ_BEGIN, p, pencil_lds_bytes<T>(a.m0, a.m1, a.m2, a.pencil_ct), st);
+}
+
+template <typename T>
+static void lines_step(const PropArgs<T>& a, int s, hipStream_t st)
+{
+    const PencilTiles p = pencil_tiles<T>(a.m0, a.m1, a.m2, a.pencil_ct);
+    const size_t lds = pencil_lds_bytes<T>(a.m0, a.m1, a.m2, a.pencil_ct);
+    lines_ly<T>(a, a.nb, 2 * a.nb, p, lds, st);
+    lines_lz<T>(a, s, p, lds, st);
+    lines_ly<T>(a, a.nb, a.nb, p, lds, st);
+    lines_lx<T>(a, s, s == a.N - 1 ? PZ_LAST : PZ_STEP, p, lds, st);
+}
+
+template <typename T>
+static void lines_resume(const PropArgs<T>& a, int c, hipStream_t st)
+{
+    const PencilTiles p = pencil_tiles<T>(a.m0, a.m1, a.m2, a.pencil_ct);
+    lines_lx<T>(a, c, PZ_RESUME, p, pencil_lds_bytes<T>(a.m0, a.m1, a.m2, a.pencil_ct), st);
+}
+
 template <typename T>
 static hipError_t upload_twiddles(cpx<T>** dst, int n)
 {
@@ -528,9 +619,10 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
     }
     // Kernel family. Default: plane/column kernels where this grid has them (the two-pass
     // step when it has that set: 16-22% faster per contour step than the four-launch
-    // two-stage step on every grid measured), the line kernels otherwise or when
-    // SCFT_PENCIL=1; SCFT_LEGACY_FFT=1 keeps the Stockham plane/column kernels;
-    // SCFT_FAMILY=two-stage|two-pass|stockham|pencil names one. Without a two-stage set, a
+    // two-stage step on every grid measured), otherwise the split line step (3-14% faster
+    // than the pencil step); SCFT_PENCIL=1 forces the pencil step, SCFT_LEGACY_FFT=1 keeps
+    // the Stockham plane/column kernels, SCFT_FAMILY=two-stage|two-pass|stockham|pencil|
+    // pencil-split names one. Without a two-stage set, a
     // Stockham plane that fills a CU's LDS runs one workgroup per CU, and the line kernels
     // are faster there (96^3 FP64: 0.529 against 0.805 ms per contour step for 8 replicas).
     int family = sc.family;
@@ -538,29 +630,32 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
         const char* fe = getenv("SCFT_FAMILY");
         if (fe && *fe) {
             const std::string f(fe);
-            family = f == "two-stage" ? SCFT_SCHED_TWO_STAGE
-                     : f == "two-pass" ? SCFT_SCHED_TWO_PASS
-                     : f == "stockham" ? SCFT_SCHED_STOCKHAM
-                     : f == "pencil"   ? SCFT_SCHED_PENCIL
-                                       : SCFT_SCHED_DEFAULT;
+            family = f == "two-stage"      ? SCFT_SCHED_TWO_STAGE
+                     : f == "two-pass"     ? SCFT_SCHED_TWO_PASS
+                     : f == "stockham"     ? SCFT_SCHED_STOCKHAM
+                     : f == "pencil"       ? SCFT_SCHED_PENCIL
+                     : f == "pencil-split" ? SCFT_SCHED_PENCIL_SPLIT
+                                           : SCFT_SCHED_DEFAULT;
         }
     }
     if (family == SCFT_SCHED_DEFAULT) {
         const char* pen = getenv("SCFT_PENCIL");
         const char* legacy = getenv("SCFT_LEGACY_FFT");
         const bool force_legacy = legacy && legacy[0] == '1';
-        if ((pen && pen[0] == '1') || !kset) family = SCFT_SCHED_PENCIL;
+        if (pen && pen[0] == '1') family = SCFT_SCHED_PENCIL;
+        else if (!kset) family = SCFT_SCHED_PENCIL_SPLIT;
         else if (kset->step3 && !force_legacy) family = SCFT_SCHED_TWO_PASS;
         else if (kset->step2 && !force_legacy) family = SCFT_SCHED_TWO_STAGE;
-        else if (!kset->step2 && kset->big_plane && !force_legacy && pencil_ok) family = SCFT_SCHED_PENCIL;
+        else if (!kset->step2 && kset->big_plane && !force_legacy && pencil_ok) family = SCFT_SCHED_PENCIL_SPLIT;
         else family = SCFT_SCHED_STOCKHAM;
     } else {
         // an explicit family must exist as such for this grid
-        const bool have = family == SCFT_SCHED_TWO_STAGE ? kset && kset->step2
-                          : family == SCFT_SCHED_TWO_PASS ? kset && kset->step3
-                          : family == SCFT_SCHED_STOCKHAM ? kset && kset->step != kset->step2
-                          : family == SCFT_SCHED_PENCIL   ? pencil_ok
-                                                          : false;
+        const bool have = family == SCFT_SCHED_TWO_STAGE    ? kset && kset->step2
+                          : family == SCFT_SCHED_TWO_PASS     ? kset && kset->step3
+                          : family == SCFT_SCHED_STOCKHAM     ? kset && kset->step != kset->step2
+                          : family == SCFT_SCHED_PENCIL       ? pencil_ok
+                          : family == SCFT_SCHED_PENCIL_SPLIT ? pencil_ok
+                                                              : false;
         if (!have) return "schedule: kernel family " + std::to_string(family) + " is not available for this grid";
     }
     hipError_t e = hipSuccess;
@@ -604,11 +699,12 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
         }
         if (sc.coltile != 0 || sc.plane_nt != 0) return "schedule: tile sizes apply to the two-stage kernels only";
         ks_ = nullptr;
-        begin_ = &pencil_begin<T>;
-        step_ = &pencil_step<T>;
-        resume_ = &pencil_resume<T>;
-        path_ = "pencil";
-        e = pencil_setup<T>();
+        const bool split = family == SCFT_SCHED_PENCIL_SPLIT;
+        begin_ = split ? &lines_begin<T> : &pencil_begin<T>;
+        step_ = split ? &lines_step<T> : &pencil_step<T>;
+        resume_ = split ? &lines_resume<T> : &pencil_resume<T>;
+        path_ = split ? "pencil-split" : "pencil";
+        e = split ? lines_setup<T>() : pencil_setup<T>();
     }
     if (e != hipSuccess) return std::string("hipFuncSetAttribute: ") + hipGetErrorString(e);
     nblk_ = (int)std::min<long long>(1024, (M_ + RED_NT - 1) / RED_NT);
@@ -675,7 +771,10 @@ std::vector<scft_schedule> PropEngine<T>::candidates(const int m[3], int nb)
             if (ct) fam.push_back({SCFT_SCHED_TWO_PASS, ct, k->plane_nt, 0});
     }
     if (k && k->step != k->step2) fam.push_back({SCFT_SCHED_STOCKHAM, 0, 0, 0});
-    if (pencil_supported(m[0], m[1], m[2])) fam.push_back({SCFT_SCHED_PENCIL, 0, 0, 0});
+    if (pencil_supported(m[0], m[1], m[2])) {
+        fam.push_back({SCFT_SCHED_PENCIL_SPLIT, 0, 0, 0});
+        fam.push_back({SCFT_SCHED_PENCIL, 0, 0, 0});
+    }
     for (scft_schedule f : fam)
         for (int ns = 1; ns <= std::min(3, std::max(1, nb)); ++ns) {
             f.streams = ns;

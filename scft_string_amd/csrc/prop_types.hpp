@@ -1,5 +1,6 @@
 // prop_types.hpp -- types shared by the propagator kernels and their host driver.
 #pragma once
+#include <type_traits>
 #include "fft_engine.hpp"
 
 namespace scft {
@@ -138,6 +139,32 @@ struct Boltz<float> {
     {
         const float v = u * (2.0f - u);
         return div3((h0 - h0 * u) * 4.0f - (h1 - h1 * v));
+    }
+};
+
+// The inverse transforms' normalisation 1/L. In FP32 a rounded 1/L (L = 96, 9216, ...)
+// scales the mean of q by (1 + d), |d| up to 3e-8, on every sandwich, the same way on every
+// step: 3N sandwiches move lnQ by ~3N d. It is applied as inv.hi + inv.lo (inv.hi = RN(1/L),
+// inv.lo = RN(1/L - inv.hi)), exact to FP32 rounding of the product. Powers of two have
+// inv.lo = 0. FP64 takes RN(1/L), whose bias (~1e-16 per sandwich) is far below its bar.
+template <typename T>
+struct InvLen {
+    T hi, lo;
+    __device__ __forceinline__ explicit InvLen(int L)
+    {
+        const double r = 1.0 / (double)L;
+        hi = (T)r;
+        lo = std::is_same<T, float>::value ? (T)(r - (double)hi) : T(0);
+    }
+    __device__ __forceinline__ void apply(cpx<T>& v) const
+    {
+        if constexpr (std::is_same<T, float>::value) {
+            v.x = fmaf(v.x, lo, v.x * hi);
+            v.y = fmaf(v.y, lo, v.y * hi);
+        } else {
+            v.x *= hi;
+            v.y *= hi;
+        }
     }
 };
 

@@ -362,7 +362,11 @@ std::string sched_name(const scft_schedule& s)
     else if (s.family == SCFT_SCHED_TWO_PASS)
         snprintf(b, sizeof b, "two-pass c%d s%d", s.coltile, s.streams);
     else
-        snprintf(b, sizeof b, "%s s%d", s.family == SCFT_SCHED_STOCKHAM ? "stockham" : "pencil", s.streams);
+        snprintf(b, sizeof b, "%s s%d",
+                 s.family == SCFT_SCHED_STOCKHAM       ? "stockham"
+                 : s.family == SCFT_SCHED_PENCIL_SPLIT ? "pencil-split"
+                                                       : "pencil",
+                 s.streams);
     return b;
 }
 
@@ -419,7 +423,7 @@ scft_status scft_ctx_create(const scft_params* p, scft_ctx** out)
     if (p->anderson < 0 || p->anderson > 40 || (p->anderson > 0 && p->dostring != 0)) return SCFT_ERR_INVALID;
     if (p->ckpt < 0) return SCFT_ERR_INVALID;
     const scft_schedule& sc = p->sched;
-    if (sc.family < SCFT_SCHED_DEFAULT || sc.family > SCFT_SCHED_TWO_PASS || sc.coltile < 0 || sc.plane_nt < 0 ||
+    if (sc.family < SCFT_SCHED_DEFAULT || sc.family > SCFT_SCHED_PENCIL_SPLIT || sc.coltile < 0 || sc.plane_nt < 0 ||
         sc.streams < 0 || sc.streams > 4)
         return SCFT_ERR_INVALID;
     // autotune = 1: the schedule comes from scft_autotune, which times every candidate in a

@@ -95,7 +95,7 @@ def main(argv=None):
     ap.add_argument("--pencil", action="store_true")
     ap.add_argument("--runtime-lengths", action="store_true")
     ap.add_argument("--fields", choices=("zero", "lamella"), default="zero")
-    ap.add_argument("--family", choices=("default", "two-stage", "two-pass", "stockham", "pencil"), default="default",
+    ap.add_argument("--family", choices=("default", "two-stage", "two-pass", "stockham", "pencil", "pencil-split"), default="default",
                     help="propagator schedule: kernel family (scft_schedule.family)")
     ap.add_argument("--streams", type=int, default=0, help="schedule: replica-group streams (0: default)")
     ap.add_argument("--coltile", type=int, default=0, help="schedule: column tile (0: default)")

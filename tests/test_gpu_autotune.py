@@ -35,14 +35,15 @@ def _cands(report):
     return re.findall(r"([a-z-]+(?: c\d+)?(?: nt\d+)? s\d) ([0-9.]+);", report)
 
 
-@pytest.mark.parametrize("m,R,prec,nfam", [((64, 64, 64), 3, "fp32", 8), ((32, 32, 32), 3, "fp64", 6),
-                                           ((16, 16, 16), 2, "fp64", 2)])
+@pytest.mark.parametrize("m,R,prec,nfam", [((64, 64, 64), 3, "fp32", 9), ((32, 32, 32), 3, "fp64", 7),
+                                           ((16, 16, 16), 2, "fp64", 3)])
 def test_autotune_reports_every_candidate_and_picks_the_fastest(m, R, prec, nfam):
     """64^3 has the two-stage kernels at two column tiles x two plane workgroup sizes, the
-    two-pass step at two column tiles, the Stockham kernels and the line kernels (8 kernel
-    choices); 32^3 the two-stage kernels at two column tiles and one plane size, the two-pass
-    step at two column tiles, Stockham, line (6); 16^3, whose axes are a single register DFT
-    and so have no two-stage set, Stockham and line (2)."""
+    two-pass step at two column tiles, the Stockham kernels and the pencil and split line
+    steps (9 kernel choices); 32^3 the two-stage kernels at two column tiles and one plane
+    size, the two-pass step at two column tiles, Stockham, two line steps (7); 16^3, whose
+    axes are a single register DFT and so have no two-stage set, Stockham and two line steps
+    (3)."""
     _say("autotune candidates", m, prec, R)
     D = (4.0, 4.0, 4.0)
     best, rep = autotune(m, D, 30, 0.45, 0.1, 1, 0.8, 15.0, nrep=R, precision=prec, dostring=0,

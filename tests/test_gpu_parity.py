@@ -233,14 +233,15 @@ def test_persistent_plane_mid_matches_per_plane(pgrid, monkeypatch):
 
 def test_96cube_fp64_stockham_and_line_kernels_agree(monkeypatch):
     """96^3 FP64 has no two-stage set, and its Stockham plane fills a CU's LDS, so it runs on
-    the line kernels by default; SCFT_LEGACY_FFT=1 keeps the Stockham kernels. Both agree."""
+    the (split) line step by default; SCFT_LEGACY_FFT=1 keeps the Stockham kernels. Both
+    agree."""
     m, D = (96, 96, 96), (8.0, 8.0, 8.0)
     W = _fields(m, D, seed=21, amp=1.5)[None]
     out = {}
     for leg in ("0", "1"):
         monkeypatch.setenv("SCFT_LEGACY_FFT", leg)
         with Context(m, D, 8, 0.5, 0.125, 2, 0.7, chi=15.0, precision="fp64") as ctx:
-            assert ctx.fft_path == ("stockham" if leg == "1" else "pencil")
+            assert ctx.fft_path == ("stockham" if leg == "1" else "pencil-split")
             out[leg] = ctx.props_batch(W)
     assert abs(out["0"][1][0] - out["1"][1][0]) <= 1e-12 * max(1.0, abs(out["1"][1][0]))
     assert _rel(out["0"][0], out["1"][0]) <= 1e-12

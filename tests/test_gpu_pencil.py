@@ -16,6 +16,7 @@ from scft_string_amd import Context
 pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-10
+PENCIL = {"family": "pencil"}  # these tests hold the pencil step; the split line step has its own
 
 
 def _say(*a):
@@ -41,7 +42,7 @@ def _rel(a, b):
 def _check64(m, D, N, f, alpha, ens, phidb, nrep, force):
     W = np.stack([_fields(m, D, seed=31 + i, mean_plus=0.3 * i) for i in range(nrep)])
     g = orc.Grid(m, D)
-    with Context(m, D, N, f, alpha, ens, phidb, chi=15.0, nrep=nrep, precision="fp64") as ctx:
+    with Context(m, D, N, f, alpha, ens, phidb, chi=15.0, nrep=nrep, precision="fp64", schedule=PENCIL) as ctx:
         assert ctx.fft_path == "pencil"
         phi, lnQ, _ = ctx.props_batch(W)
     for i in range(nrep):
@@ -89,7 +90,7 @@ def test_pencil_fp32_error_vs_oracle(m):
     N, f = 30, 0.5
     W = _fields(m, D, seed=8)[None]
     ref = orc.Grid(m, D).props(W[0], N=N, alpha=0.1, phic=0.8, f=f, ens=1)
-    with Context(m, D, N, f, 0.1, 1, 0.8, chi=15.0, precision="fp32") as ctx:
+    with Context(m, D, N, f, 0.1, 1, 0.8, chi=15.0, precision="fp32", schedule=PENCIL) as ctx:
         assert ctx.fft_path == "pencil"
         phi, lnQ, _ = ctx.props_batch(W)
     dl = abs(lnQ[0] - ref["lnQ"])
@@ -128,9 +129,10 @@ def test_pencil_and_plane_column_agree(prec, tol, monkeypatch):
     assert dl <= tol and dphi <= 10 * tol
 
 
+@pytest.mark.parametrize("fam", ["pencil", "pencil-split"])
 @pytest.mark.parametrize("m", [(128, 150, 160), (150, 12, 200), (96, 20, 96)])
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
-def test_compile_time_line_kernels_match_runtime(m, prec, tol, monkeypatch):
+def test_compile_time_line_kernels_match_runtime(m, prec, tol, fam, monkeypatch):
     """Axes of lengths 96, 128, 150, 160, 192, 200, 256 run line kernels compiled for that length
     (fixed tiles, unrolled passes); SCFT_PENCIL_RT=1 forces the runtime-length plan. Both
     compute the same transform: props and the box gradient must agree."""
@@ -140,24 +142,26 @@ def test_compile_time_line_kernels_match_runtime(m, prec, tol, monkeypatch):
     for rt in ("0", "1"):
         monkeypatch.setenv("SCFT_PENCIL_RT", rt)
         _say(f"line kernels {m} {prec} runtime={rt}")
-        with Context(m, D, 12, 0.45, 0.1, 1, 0.9, chi=18.0, nrep=2, precision=prec) as ctx:
-            assert ctx.fft_path == "pencil"
+        with Context(m, D, 12, 0.45, 0.1, 1, 0.9, chi=18.0, nrep=2, precision=prec, schedule={"family": fam}) as ctx:
+            assert ctx.fft_path == fam
             phi, lnQ, _ = ctx.props_batch(W)
             _, dq = ctx.dlnq_dbox(W)
             out[rt] = (phi, lnQ, dq)
     dl = float(np.max(np.abs(out["0"][1] - out["1"][1])))
     dphi = _rel(out["0"][0], out["1"][0])
     dd = _rel(out["0"][2], out["1"][2])
-    _say(f"compile-time vs runtime {m} {prec}: |dlnQ| {dl:.2e}, rel dphi {dphi:.2e}, rel d(dlnQ/dD) {dd:.2e}")
+    _say(f"compile-time vs runtime {fam} {m} {prec}: |dlnQ| {dl:.2e}, rel dphi {dphi:.2e}, rel d(dlnQ/dD) {dd:.2e}")
     assert dl <= tol and dphi <= 10 * tol and dd <= 100 * tol
 
 
-def test_fp64_128_cube_vs_oracle():
-    """128^3 FP64: a plane does not fit in LDS, so this grid runs on the line kernels."""
+@pytest.mark.parametrize("fam", ["pencil", "default"])
+def test_fp64_128_cube_vs_oracle(fam):
+    """128^3 FP64: a plane does not fit in LDS, so this grid runs on the line kernels (by
+    default the split line step)."""
     m, D, N = (128, 128, 128), (8.0, 8.0, 8.0), 16
     W = _fields(m, D, seed=12)[None]
-    with Context(m, D, N, 0.45, 0.1, 1, 0.9, chi=18.0, precision="fp64") as ctx:
-        assert ctx.fft_path == "pencil"
+    with Context(m, D, N, 0.45, 0.1, 1, 0.9, chi=18.0, precision="fp64", schedule={"family": fam}) as ctx:
+        assert ctx.fft_path == ("pencil" if fam == "pencil" else "pencil-split")
         phi, lnQ, _ = ctx.props_batch(W)
     ref = orc.Grid(m, D).props(W[0], N=N, alpha=0.1, phic=0.9, f=0.45, ens=1)
     assert abs(lnQ[0] - ref["lnQ"]) <= TOL64 * max(1.0, abs(ref["lnQ"]))
@@ -167,12 +171,13 @@ def test_fp64_128_cube_vs_oracle():
 
 def test_shipped_input_dat_box_fp64_vs_oracle():
     """The reference's own case: m = 200 150 150, D = 16 12 12, N = 60, ens = 2,
-    phidb = 1.16e-5, f = 0.8, chi = 30 (input.dat:1-11), one replica, FP64."""
+    phidb = 1.16e-5, f = 0.8, chi = 30 (input.dat:1-11), one replica, FP64, on the default
+    (split) line step."""
     m, D, N = (200, 150, 150), (16.0, 12.0, 12.0), 60
     _say("input.dat box: GPU props")
     W = _fields(m, D, seed=2, amp=1.0)[None]
     with Context(m, D, N, 0.8, 0.1, 2, 1.16e-5, chi=30.0, precision="fp64") as ctx:
-        assert ctx.fft_path == "pencil"
+        assert ctx.fft_path == "pencil-split"
         phi, lnQ, _ = ctx.props_batch(W)
     _say("input.dat box: oracle props")
     orc.set_threads(16)

@@ -118,7 +118,7 @@ CASES = [
     # m, D, N, f, alpha, ens, phidb, kernel family
     ((32, 32, 32), (4.0, 4.0, 4.0), 40, 0.45, 0.1, 1, 0.9, "two-pass"),
     ((64, 64, 64), (6.0, 6.0, 6.0), 128, 0.45, 0.1, 1, 0.9, "two-pass"),
-    ((20, 15, 15), (2.0, 1.5, 1.5), 30, 0.8, 0.2, 2, 0.3, "pencil"),
+    ((20, 15, 15), (2.0, 1.5, 1.5), 30, 0.8, 0.2, 2, 0.3, "pencil-split"),
     ((12, 8, 16), (3.0, 2.5, 4.0), 25, 0.4, 0.1, 1, 0.7, None),
 ]
 
