@@ -941,9 +941,9 @@ template <typename T>
 hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out, hipStream_t st)
 {
     if (nb <= 0) return hipSuccess;
-    // the two-stage gradient passes where the props run on the two-stage kernels; the line
-    // pipeline otherwise (any 2-3-5 grid)
-    const bool two = ks_ && ks_->gterm2 && step_ == ks_->step2;
+    // the two-stage gradient passes where the props run on the two-stage or the two-pass step
+    // (the same discrete operator, to rounding); the line pipeline otherwise (any 2-3-5 grid)
+    const bool two = ks_ && ks_->gterm2 && (step_ == ks_->step2 || (ks_->step3 && step_ == ks_->step3));
     if (!two && !pencil_supported(m_[0], m_[1], m_[2])) return hipErrorNotSupported;
     hipError_t e;
     if (two && (e = ks_->gsetup2()) != hipSuccess) return e;
