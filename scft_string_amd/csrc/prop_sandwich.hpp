@@ -122,13 +122,19 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT
             }
         }
     }
+    // f is loaded before the second h sandwich so that its reads overlap that sandwich where
+    // the registers allow: the extra field spills at 128^2 FP32 and costs a wave per SIMD at
+    // 96^2 FP32, where the later load is faster
+    constexpr bool early_f = !(std::is_same<T, float>::value && NE > 64 * 64);
+    cpx<T> zf[IZ1][A2];
+    if (early_f) sp_load<P>(a.Cf + fo, zf, tid);
     __syncthreads();
     sp_yz_sandwich<P>(z, buf, twz, twy, gy, gz, false, inv, tid);
     sp_store<P>(a.Ch + fo, z, tid);
-    sp_load<P>(a.Cf + fo, z, tid);
+    if (!early_f) sp_load<P>(a.Cf + fo, zf, tid);
     __syncthreads();
-    sp_yz_sandwich<P>(z, buf, twz, twy, gy, gz, true, inv, tid);
-    sp_store<P>(a.Cf + fo, z, tid);
+    sp_yz_sandwich<P>(zf, buf, twz, twy, gy, gz, true, inv, tid);
+    sp_store<P>(a.Cf + fo, zf, tid);
 }
 
 // x-sandwich of a tile of TC x-lines in stage-1 ownership (item (c, n1) holds
@@ -326,10 +332,10 @@ k_scol2(PropArgs<T> a, int s, int mode)
         __syncthreads();  // tables
     } else {
         load(a.Ch + so, q);  // q holds h until the combination below
+        cpx<T> f[I1][X2];    // f is read before the h sandwich, which its reads overlap
+        load(a.Cf + so, f);
         __syncthreads();     // tables
         sc_x_sandwich<G, T, N0, TC>(q, buf, tw0, gx, false, inv, dsum, tid);
-        cpx<T> f[I1][X2];
-        load(a.Cf + so, f);
         load_factors();
         const bool qa = s < a.n1, pb = s < a.n2;  // species of step s (density.h:136-146)
 #pragma unroll
