@@ -20,8 +20,14 @@
 //   k_gx(FWD)   A: forward x -> K1
 //   k_gacc      sum_k dK12/dD_i(k) Im(K0(-k) K1(k)) and dK6/dD_i(k) Im(Kf(-k) Kf(k))
 // with dK12/dD_i = K12 k_i^2 ds / (6 D_i) and dK6/dD_i = K6 k_i^2 ds / (3 D_i), from
-// k^2 = sum_j (2 pi n_j / D_j)^2 (density.h:79-95).
+// k^2 = sum_j (2 pi n_j / D_j)^2 (density.h:79-95). That is the line pipeline (any 2-3-5
+// grid). Grids with a two-stage kernel set take four passes per step instead:
+//   k_gplane_init2   Z0, Zf from the q store and E4, E2 -> (y, z) spectra   (A, B)
+//   k_gcol2(SPLIT)   A: forward x -> K0, times K12, inverse x -> A;  B: forward x -> Kf
+//   k_plane_mid2     A: inverse (y, z), times E2, forward (y, z)
+//   k_gcol2_acc      A: forward x -> K1 in registers, the sum over k with K0, Kf read at +-k
 #pragma once
+#include "prop_kernels2.hpp"
 #include "prop_pencil.hpp"
 
 namespace scft {
@@ -187,13 +193,6 @@ __global__ void __launch_bounds__(PC_NT) k_gx(PropArgs<T> a, int mode, int nh, i
     }
 }
 
-// Box-derivative constants: ds and 2 pi / D_i.
-struct GradConst {
-    double ds;
-    double kD[3];  // 2 pi / D_i
-    double D[3];
-};
-
 // Adds, per slot and direction i, sum_k [4/3 dK12_i(k) Im(K0(-k) K1(k)) - 1/6 dK6_i(k)
 // Im(Kf(-k) Kf(k))] to part[slot][blk][3] (scaled by M^-2 on the host), where
 // dK12_i = exp(-k^2 ds/12) k_i^2 ds / (6 D_i) and dK6_i = exp(-k^2 ds/6) k_i^2 ds / (3 D_i).
@@ -247,6 +246,123 @@ __global__ void __launch_bounds__(RED_NT) k_gacc(const cpx<T>* __restrict__ K0, 
         double v = 0;
         for (int w = 0; w < RED_NT / 64; ++w) v += wsum[threadIdx.x][w];
         part[((size_t)slot * gridDim.x + blockIdx.x) * 3 + threadIdx.x] += v;
+    }
+}
+
+// Last pass of a box-gradient term on the two-stage FFT, fused with the reduction over k.
+// Ch of each slot holds the (y, z) spectrum of Z1 = E1/2 K12 Z0 with x in natural order; the
+// forward FFT along x gives K1(k) on the workgroup's TC columns, which is paired in registers
+// with K0(-k), Kf(-k) and Kf(k) (written by k_gcol2 GC_SPLIT) to add, per slot and
+// direction i, the columns' share of
+//   sum_k [4/3 dK12_i(k) Im(K0(-k) K1(k)) - 1/6 dK6_i(k) Im(Kf(-k) Kf(k))]
+// to part[slot][tile][i] (scaled by M^-2 on the host), the sum k_gacc forms from stored
+// spectra. K1 is never stored. grid (ntile, nb), block C2Geom<N0, TC>::NT. A slot's launches
+// run in order on one stream, so each workgroup's read-modify-write of its partial is
+// deterministic.
+template <typename T, int N0, int NY, int NZ, int TC>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<N0, TC>::NT)))
+k_gcol2_acc(PropArgs<T> a, const cpx<T>* __restrict__ K0, const cpx<T>* __restrict__ Kf, GradConst gc,
+            double* __restrict__ part)
+{
+    using G = C2Geom<N0, TC>;
+    constexpr int NT = G::NT, X1 = G::X1, X2 = G::X2, I1 = G::I1, I2 = G::I2, CX = G::CX, PX = G::PX;
+    constexpr int NC = NY * NZ;
+    static_assert(NT % 64 == 0, "whole waves");
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* tw0 = buf + G::LDSN;
+    const int tid = threadIdx.x;
+    const int tile = blockIdx.x, slot = blockIdx.y;
+    const size_t so = (size_t)slot * (size_t)a.M;
+    const cpx<T>* fld = a.Ch + so;
+    const cpx<T>* k0 = K0 + so;
+    const cpx<T>* kf = Kf + so;
+    const int c0 = tile * TC;
+    for (int i = tid; i < N0; i += NT) tw0[i] = a.tw0[i];
+    // stage 1: rows n1 + X1 n2 of each column, forward along n2, twiddled into LDS
+    cpx<T> v[I1][X2];
+#pragma unroll
+    for (int it = 0; it < I1; ++it) {
+        const int w = tid + it * NT;
+        if ((TC * X1) % NT == 0 || w < TC * X1) {
+            const int c = w % TC, n1 = w / TC, col = c0 + c;
+            const bool in = NC % TC == 0 || col < NC;
+            sfor<X2>([&](auto nc) {
+                constexpr int n2 = decltype(nc)::value;
+                v[it][n2] = in ? fld[(size_t)(n1 + X1 * n2) * NC + col] : cpx<T>{T(0), T(0)};
+            });
+        }
+    }
+    __syncthreads();  // twiddle table
+#pragma unroll
+    for (int it = 0; it < I1; ++it) {
+        const int w = tid + it * NT;
+        if ((TC * X1) % NT == 0 || w < TC * X1) {
+            const int c = w % TC, n1 = w / TC;
+            rdft<T, X2, false>(v[it]);
+            sfor<X2>([&](auto kc) {
+                constexpr int k2 = decltype(kc)::value;
+                buf[c * CX + n1 * PX + k2] = k2 == 0 ? v[it][0] : cmul(v[it][k2], tw0[(n1 * k2) % N0]);
+            });
+        }
+    }
+    __syncthreads();
+    // stage 2: forward along n1 gives K1 at kx = k2 + X2 k1; each point meets its mirror
+    double t[3] = {0, 0, 0};
+#pragma unroll
+    for (int it = 0; it < I2; ++it) {
+        const int w = tid + it * NT;
+        if ((TC * X2) % NT == 0 || w < TC * X2) {
+            const int c = w % TC, k2 = w / TC, col = c0 + c;
+            cpx<T> u[X1];
+            sfor<X1>([&](auto nc) {
+                constexpr int n1 = decltype(nc)::value;
+                u[n1] = buf[c * CX + n1 * PX + k2];
+            });
+            rdft<T, X1, false>(u);
+            if (NC % TC == 0 || col < NC) {
+                const int ky = col / NZ, kz = col - ky * NZ;
+                const int ny = ky <= NY / 2 ? ky : ky - NY, nz = kz <= NZ / 2 ? kz : kz - NZ;
+                const int colm = (ky == 0 ? 0 : NY - ky) * NZ + (kz == 0 ? 0 : NZ - kz);
+                const double k2y = (gc.kD[1] * ny) * (gc.kD[1] * ny), k2z = (gc.kD[2] * nz) * (gc.kD[2] * nz);
+                sfor<X1>([&](auto kc) {
+                    constexpr int k1 = decltype(kc)::value;
+                    const int kx = k2 + X2 * k1;
+                    const int nx = kx <= N0 / 2 ? kx : kx - N0, kxm = kx == 0 ? 0 : N0 - kx;
+                    const double k2x = (gc.kD[0] * nx) * (gc.kD[0] * nx);
+                    // every point reads its own Kf and its mirror's: taking each +-k pair once
+                    // (the term is even) from half the columns made the pass slower
+                    const cpx<T> a0 = k0[(size_t)kxm * NC + colm], fm = kf[(size_t)kxm * NC + colm],
+                                 fp = kf[(size_t)kx * NC + col];
+                    const double bh = (double)a0.x * (double)u[k1].y + (double)a0.y * (double)u[k1].x;
+                    const double bf = (double)fm.x * (double)fp.y + (double)fm.y * (double)fp.x;
+                    const double kh = exp(-(k2x + k2y + k2z) * gc.ds / 12.0);
+                    // 4/3 * ds/(6 D_i) * kh * bh - 1/3 * ds/(3 D_i) * kh^2 * bf / 2, times k_i^2
+                    const double cf = gc.ds * ((2.0 / 9.0) * kh * bh - (1.0 / 18.0) * kh * kh * bf);
+                    t[0] += cf * k2x;
+                    t[1] += cf * k2y;
+                    t[2] += cf * k2z;
+                });
+            }
+        }
+    }
+    // the 1/D_i once per thread; each wave sums its lanes by shuffles, and the wave sums
+    // meet in LDS (buf is free once every wave has read its stage-2 items)
+    __syncthreads();
+    double* wsum = reinterpret_cast<double*>(smem_raw);
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double x = t[i] / gc.D[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+        if (lane == 0) wsum[i * (NT / 64) + wave] = x;
+    }
+    __syncthreads();
+    if (tid < 3) {
+        double x = 0;
+        for (int w = 0; w < NT / 64; ++w) x += wsum[tid * (NT / 64) + w];
+        part[((size_t)slot * gridDim.x + tile) * 3 + tid] += x;
     }
 }
 

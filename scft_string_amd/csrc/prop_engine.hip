@@ -244,28 +244,41 @@ struct KS {
             hipError_t e = hipFuncSetAttribute((const void*)k_gplane_init2<T, N1, N2>,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2_plane);
             if (e != hipSuccess) return e;
+            e = hipFuncSetAttribute((const void*)k_gcol2_acc<T, N0, N1, N2, TC>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2_col);
+            if (e != hipSuccess) return e;
             return hipFuncSetAttribute((const void*)k_gcol2<T, N0, N1, N2, TC>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2_col);
         }
         return hipErrorNotSupported;
     }
-    // the four passes of one gradient term over a.nb slots (the fifth, k_gacc, is the
-    // caller's): Ch, Cf <- F2(E1/4 Z), F2(E1/2 Z); x: K0, Kf <- spectra, Ch <- K12 sandwich;
-    // Ch <- F2(E1/2 F2^-1 Ch) (plane_mid with both parts on species spec); x: K1 <- spectrum
-    static void gterm_two(const PropArgs<T>& a, int ib, int pbk, int ia, int pak, int spec, cpx<T>* K0,
-                          cpx<T>* K1, cpx<T>* Kf, hipStream_t st)
+    // the passes of one gradient term, or of a term and its mirror (specB >= 0), over a.nb
+    // slots: Gh, Gf <- F2(E1/4 Z), F2(E1/2 Z); x: K0, Kf <- spectra, Gh <- K12 sandwich;
+    // Gh <- F2(E1/2 F2^-1 Gh) (plane_mid with both parts on the term's species, once per
+    // term: the factors are the replica's); x: K1 spectrum paired with K0, Kf and summed
+    static void gterm_two(const PropArgs<T>& a, int ib, int pbk, int ia, int pak, int specA, int specB,
+                          cpx<T>* Gh, cpx<T>* Gf, cpx<T>* K0, cpx<T>* Kf, const GradConst& gc, double* part,
+                          hipStream_t st)
     {
         if constexpr (two_ok) {
-            hipLaunchKernelGGL((k_gplane_init2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, a, ib, pbk,
-                               ia, pak, spec);
-            hipLaunchKernelGGL((k_gcol2<T, N0, N1, N2, TC>), dim3(ntile, 2 * a.nb), dim3(CG::NT), lds2_col, st, a,
-                               a.nb, (int)GC_SPLIT, K0, Kf);
-            PropArgs<T> b = a;  // k_plane_mid2 takes the real part's species from s < n1, the
-            b.n1 = spec == 0 ? 1 : 0;  // imaginary part's from s < n2: at s = 0 both get spec
-            b.n2 = spec == 0 ? 0 : 1;
-            hipLaunchKernelGGL((k_plane_mid2<T, N1, N2>), dim3(N0, a.nb), dim3(PG::NT), lds2_plane, st, b, 0);
-            hipLaunchKernelGGL((k_gcol2<T, N0, N1, N2, TC>), dim3(ntile, a.nb), dim3(CG::NT), lds2_col, st, a, a.nb,
-                               (int)GC_FWD, K1, (cpx<T>*)nullptr);
+            const int nb = a.nb, nterm = specB >= 0 ? 2 : 1, nt = nterm * nb;
+            PropArgs<T> w = a;
+            w.Ch = Gh;
+            w.Cf = Gf;
+            hipLaunchKernelGGL((k_gplane_init2<T, N1, N2>), dim3(N0, nb), dim3(PG::NT), lds2_plane, st, w, ib, pbk, ia,
+                               pak, specA, specB);
+            hipLaunchKernelGGL((k_gcol2<T, N0, N1, N2, TC>), dim3(ntile, 2 * nt), dim3(CG::NT), lds2_col, st, w, nt,
+                               (int)GC_SPLIT, K0, Kf);
+            for (int t = 0; t < nterm; ++t) {
+                PropArgs<T> b = a;  // k_plane_mid2 takes the real part's species from s < n1, the
+                b.Ch = Gh + (size_t)t * nb * a.M;  // imaginary part's from s < n2: at s = 0 both
+                const int spec = t ? specB : specA;  // get spec
+                b.n1 = spec == 0 ? 1 : 0;
+                b.n2 = spec == 0 ? 0 : 1;
+                hipLaunchKernelGGL((k_plane_mid2<T, N1, N2>), dim3(N0, nb), dim3(PG::NT), lds2_plane, st, b, 0);
+            }
+            hipLaunchKernelGGL((k_gcol2_acc<T, N0, N1, N2, TC>), dim3(ntile, nt), dim3(CG::NT), lds2_col, st, w,
+                               (const cpx<T>*)K0, (const cpx<T>*)Kf, gc, part);
         }
     }
     static KernelSet<T> make()
@@ -289,6 +302,7 @@ struct KS {
             k.resume2 = &resume_two;
             k.gsetup2 = &gsetup_two;
             k.gterm2 = &gterm_two;
+            k.gtiles = ntile;
         }
         if (!legacy_ok) {
             k.setup = &setup_two;
@@ -439,7 +453,7 @@ template <typename T>
 PropEngine<T>::~PropEngine()
 {
     void* ptrs[] = {Ch_,   Cf_,   e4A_,  e4B_,  qs_, tw_[0], tw_[1], tw_[2], g_[0],
-                    g_[1], g_[2], part_, cAB_, scal_, K0_, K1_,    Kf_,    gpart_, rn_};
+                    g_[1], g_[2], part_, cAB_, scal_, K0_, K1_,    Kf_,    gpart_, rn_, Gh_, Gf_};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : ev_)
@@ -935,8 +949,8 @@ PropArgs<T> PropEngine<T>::args(int nb, const ChainConst& cc) const
 }
 
 // Box derivative of the factored QC', Q2' (adjoint of props; the pipeline and its maths
-// are described in grad_kernels.hpp). Runs on the line kernels whatever path run() took:
-// the q store and the Boltzmann factors have the same layout on every path.
+// are described in grad_kernels.hpp). The q store and the Boltzmann factors have the same
+// layout on every path, so either gradient pipeline reads what any props path stored.
 template <typename T>
 hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out, hipStream_t st)
 {
@@ -947,29 +961,48 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
     if (!two && !pencil_supported(m_[0], m_[1], m_[2])) return hipErrorNotSupported;
     hipError_t e;
     if (two && (e = ks_->gsetup2()) != hipSuccess) return e;
+    // partials per slot and chain: the column tiles of k_gcol2_acc, or k_gacc's blocks
+    const int gt = two ? ks_->gtiles : 0;
     if (!gpart_) {
+        // the two-stage pipeline carries a term and its mirror: work fields of 2 nbmax slots
         const size_t fld = sizeof(cpx<T>) * (size_t)M_ * nbmax_;
-        const size_t pb = sizeof(double) * (size_t)nbmax_ * 3 * nblk_ * 3;
-        if ((e = hipMalloc((void**)&K0_, fld)) != hipSuccess || (e = hipMalloc((void**)&K1_, fld)) != hipSuccess ||
-            (e = hipMalloc((void**)&Kf_, fld)) != hipSuccess || (e = hipMalloc((void**)&gpart_, pb)) != hipSuccess) {
-            (void)hipGetLastError();  // not left for the next launch check to report
-            return e;
+        const size_t pb = sizeof(double) * (size_t)nbmax_ * 2 * 3 * std::max(nblk_, gt) * 3;
+        if (two) {
+            if ((e = hipMalloc((void**)&Gh_, 2 * fld)) != hipSuccess ||
+                (e = hipMalloc((void**)&Gf_, 2 * fld)) != hipSuccess ||
+                (e = hipMalloc((void**)&K0_, 2 * fld)) != hipSuccess ||
+                (e = hipMalloc((void**)&Kf_, 2 * fld)) != hipSuccess || (e = hipMalloc((void**)&gpart_, pb)) != hipSuccess) {
+                (void)hipGetLastError();  // not left for the next launch check to report
+                return e;
+            }
+            bytes_ += 8 * fld + pb;
+        } else {
+            if ((e = hipMalloc((void**)&K0_, fld)) != hipSuccess || (e = hipMalloc((void**)&K1_, fld)) != hipSuccess ||
+                (e = hipMalloc((void**)&Kf_, fld)) != hipSuccess || (e = hipMalloc((void**)&gpart_, pb)) != hipSuccess) {
+                (void)hipGetLastError();
+                return e;
+            }
+            bytes_ += 3 * fld + pb;
+            const int lds = (int)pencil_lds_max<T>();
+            const void* fns[] = {(const void*)k_gz<T>, (const void*)k_gx<T>, (const void*)k_py<T, false>,
+                                 (const void*)k_py<T, true>};
+            for (const void* f : fns)
+                if ((e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds)) != hipSuccess)
+                    return e;
         }
-        bytes_ += 3 * fld + pb;
-        const int lds = (int)pencil_lds_max<T>();
-        const void* fns[] = {(const void*)k_gz<T>, (const void*)k_gx<T>, (const void*)k_py<T, false>,
-                             (const void*)k_py<T, true>};
-        for (const void* f : fns)
-            if ((e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds)) != hipSuccess) return e;
     }
-    const size_t np = (size_t)nb * nblk_ * 3;  // partials per chain
     // k_gacc workgroups per slot: about 1024 over the launch (4 per CU), so that many
     // replicas take several points per thread and the block sums stay few (one point per
     // thread over 8 x 1024 workgroups cost 23 us per launch at 8 x 64^3, 128 workgroups
     // per slot 17 us; one replica on 128 workgroups left half the CUs idle)
-    const int gblk = std::max(1, std::min(nblk_, std::max(128, 1024 / std::max(nb, 1))));
+    const int gblk = two ? gt : std::max(1, std::min(nblk_, std::max(128, 1024 / std::max(nb, 1))));
+    const int nslot = two ? 2 * nb : nb;            // mirror terms add into slots nb .. 2nb-1
+    const size_t np = (size_t)nslot * gblk * 3;  // partials per chain
     if ((e = hipMemsetAsync(gpart_, 0, sizeof(double) * 3 * np, st)) != hipSuccess) return e;
 
+    // One launch per pass over all slots on st: at 64^3 the passes take 8-16 us each, and
+    // replica groups on their own streams (as the props run) doubled the launches per term
+    // and left the groups' kernels alternating rather than overlapping.
     const PropArgs<T> a = args(nb, cc);
     // pencil_py runs on the props tiling (compiled lengths where they exist); k_gz and k_gx
     // are runtime-length kernels sized for PC_EMAX elements, so they take the runtime tiling.
@@ -987,16 +1020,12 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
     //   chain 0: b = q1[s], a = q2[N-1-s], species of chain 1 on step s (density.h:136-146)
     //   chain 1: b = q2[t], a = q2[Nh-1-t], species of chain 2 on step t
     //   chain 2: b = q1[t], a = q1[Nh-1-t], species of chain 1 on step t (A for Nh <= n1)
+    // and its mirror m = L-1-s (L = N or Nh) reads the same entries the other way round.
+    auto species = [&](int chain, int s) { return chain == 1 ? (s < cc.n2 ? 1 : 0) : (s < cc.n1 ? 0 : 1); };
     auto term = [&](int chain, int s, int ib, int ia) {
         double* pc = gpart_ + (size_t)chain * np;
         const int pbk = chain == 1 ? 1 : 0, pak = chain == 2 ? 0 : 1;
-        const int spec = chain == 1 ? (s < cc.n2 ? 1 : 0) : (s < cc.n1 ? 0 : 1);
-        if (two) {
-            ks_->gterm2(a, ib, pbk, ia, pak, spec, K0_, K1_, Kf_, st);
-            hipLaunchKernelGGL((k_gacc<T>), dim3(gblk, nb), dim3(RED_NT), 0, st, K0_, K1_, Kf_, M_, m_[0], m_[1],
-                               m_[2], gc, pc);
-            return;
-        }
+        const int spec = species(chain, s);
         hipLaunchKernelGGL((k_gz<T>), gzg, dim3(PC_NT), lds, st, a, (int)GZ_INIT, ib, pbk, ia, pak, spec, pg.nlz);
         pencil_py<T, false>(a, nb, 2 * nb, p, lds, st);
         hipLaunchKernelGGL((k_gx<T>), dim3(m_[1] * ntx, 2 * nb), dim3(PC_NT), lds, st, a, (int)GX_SPLIT, nb, pg.tx,
@@ -1009,12 +1038,22 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
         hipLaunchKernelGGL((k_gacc<T>), dim3(gblk, nb), dim3(RED_NT), 0, st, K0_, K1_, Kf_, M_, m_[0], m_[1],
                            m_[2], gc, pc);
     };
+    // terms s and m = L-1-s, from entries is (index s) and im (index m)
+    auto pair = [&](int chain, int s, int m, int is, int im) {
+        if (two) {
+            const int pbk = chain == 1 ? 1 : 0, pak = chain == 2 ? 0 : 1;
+            ks_->gterm2(a, is, pbk, im, pak, species(chain, s), m != s ? species(chain, m) : -1, Gh_, Gf_, K0_, Kf_,
+                        gc, gpart_ + (size_t)chain * np, st);
+            return;
+        }
+        term(chain, s, is, im);
+        if (m != s) term(chain, m, im, is);
+    };
     const bool hA = cc.phihA != 0.0;  // chain 2 (the A homopolymer's Q1) only when it is present
     if (ck_ == 0) {
-        for (int s = 0; s < cc.N; ++s) term(0, s, s, cc.N - 1 - s);
-        for (int t = 0; t < cc.Nh; ++t) term(1, t, t, cc.Nh - 1 - t);
-        if (hA)
-            for (int t = 0; t < cc.Nh; ++t) term(2, t, t, cc.Nh - 1 - t);
+        for (int s = 0; 2 * s <= cc.N - 1; ++s) pair(0, s, cc.N - 1 - s, s, cc.N - 1 - s);
+        for (int chain = 1; chain <= (hA ? 2 : 1); ++chain)
+            for (int t = 0; 2 * t <= cc.Nh - 1; ++t) pair(chain, t, cc.Nh - 1 - t, t, cc.Nh - 1 - t);
     } else {
         // Terms s and L-1-s (L = N for chain 0, Nh for chains 1, 2) read the same two
         // contour indices: window 0 holds s in [lo, hi), window 1 the mirrors (L-1-hi, L-1-lo].
@@ -1028,10 +1067,8 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
                 window(aw, 1, L - hi, L - 1 - lo, st);
                 for (int s = lo; s < hi; ++s) {
                     const int m = L - 1 - s, is = w0 + (s - lo), im = w1 + (m - (L - hi));
-                    for (int chain = kind == 0 ? 0 : 1; chain <= (kind == 0 ? 0 : (hA ? 2 : 1)); ++chain) {
-                        term(chain, s, is, im);
-                        if (m != s) term(chain, m, im, is);
-                    }
+                    for (int chain = kind == 0 ? 0 : 1; chain <= (kind == 0 ? 0 : (hA ? 2 : 1)); ++chain)
+                        pair(chain, s, m, is, im);
                 }
             }
         }
@@ -1046,7 +1083,8 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
         for (int chain = 0; chain < 3; ++chain)
             for (int i = 0; i < 3; ++i) {
                 double v = 0;
-                for (int b = 0; b < gblk; ++b) v += h[(size_t)chain * np + ((size_t)slot * gblk + b) * 3 + i];
+                for (int sl = slot; sl < nslot; sl += nb)  // the slot's terms, then its mirror terms
+                    for (int b = 0; b < gblk; ++b) v += h[(size_t)chain * np + ((size_t)sl * gblk + b) * 3 + i];
                 out[(size_t)slot * 9 + chain * 3 + i] = v * scale;
             }
     return hipSuccess;

@@ -31,12 +31,14 @@ struct KernelSet {
     int tc, tc_alt;
     // the two-stage set has the persistent plane_mid (k_plane_mid2p)
     bool persist_mid;
-    // one box-gradient term on the two-stage kernels (k_gplane_init2, k_gcol2, k_plane_mid2):
-    // b, a from q-store entries ib (part pbk), ia (part pak), species spec; spectra to K0, K1,
-    // Kf (nullptr when the grid has no two-stage set)
+    // one box-gradient term on the two-stage kernels (k_gplane_init2, k_gcol2, k_plane_mid2,
+    // k_gcol2_acc), or with specB >= 0 a term and its mirror: b, a from q-store entries ib
+    // (part pbk), ia (part pak), species specA (the mirror: entries swapped, species specB);
+    // work fields Gh, Gf, K0, Kf of 2 nb slots; gtiles partials per slot (mirror: slots
+    // nb .. 2nb-1) added to part (nullptr when the grid has no two-stage set)
     hipError_t (*gsetup2)();
-    void (*gterm2)(const PropArgs<T>&, int ib, int pbk, int ia, int pak, int spec, cpx<T>* K0, cpx<T>* K1,
-                   cpx<T>* Kf, hipStream_t);
+    void (*gterm2)(const PropArgs<T>&, int ib, int pbk, int ia, int pak, int specA, int specB, cpx<T>* Gh,
+                   cpx<T>* Gf, cpx<T>* K0, cpx<T>* Kf, const GradConst& gc, double* part, hipStream_t);
     // two-stage plane kernels: the default workgroup size and the alternative (0: none)
     int plane_nt, plane_nt_alt;
     // the two-pass step (prop_sandwich.hpp; nullptr when the grid has none) and its other
@@ -46,6 +48,7 @@ struct KernelSet {
     void (*begin3)(const PropArgs<T>&, hipStream_t);
     void (*step3)(const PropArgs<T>&, int s, hipStream_t);
     void (*resume3)(const PropArgs<T>&, int c, hipStream_t);
+    int gtiles;
 };
 
 // kernel set for an (m0, m1, m2) grid, or nullptr when no instantiation exists
@@ -161,6 +164,8 @@ private:
     T* g_[3] = {nullptr, nullptr, nullptr};
     double* part_ = nullptr;
     // box-gradient spectra and partial sums (allocated on first use)
+    cpx<T>* Gh_ = nullptr;  // two-stage gradient work fields (2 nbmax slots each)
+    cpx<T>* Gf_ = nullptr;
     cpx<T>* K0_ = nullptr;
     cpx<T>* K1_ = nullptr;
     cpx<T>* Kf_ = nullptr;

@@ -580,11 +580,15 @@ k_plane_end2(PropArgs<T> a, int s, int mode)
 
 // First pass of one box-gradient term (grad_kernels.hpp) on the two-stage FFT: b = part pbk
 // of q-store entry ib and a = part pak of entry ia (0 real, 1 imaginary), both taking the
-// step operator of species spec (0 A, 1 B), packed as Z = b + i a;
-// Ch <- F2(E1/4 Z), Cf <- F2(E1/2 Z).
+// step operator of species specA (0 A, 1 B), packed as Z = b + i a;
+// Ch <- F2(E1/4 Z), Cf <- F2(E1/2 Z) for slot y. With specB >= 0 the workgroup also forms
+// the mirror term, which reads the same two entries with their roles swapped (b = part pbk
+// of entry ia, a = part pak of entry ib, species specB), into slot y + nb of Ch and Cf; its
+// second read of the entries follows the first in the same workgroup, so it is served from
+// cache. The factors are those of slot y for both terms.
 template <typename T, int NY, int NZ>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT)))
-k_gplane_init2(PropArgs<T> a, int ib, int pbk, int ia, int pak, int spec)
+k_gplane_init2(PropArgs<T> a, int ib, int pbk, int ia, int pak, int specA, int specB)
 {
     using P = P2<T, NY, NZ>;
     constexpr int NT = P::NT, IZ1 = P::IZ1, A2 = P::A2, NE = NY * NZ;
@@ -596,30 +600,36 @@ k_gplane_init2(PropArgs<T> a, int ib, int pbk, int ia, int pak, int spec)
     const int x = blockIdx.x, slot = blockIdx.y;
     const size_t fo = (size_t)slot * (size_t)a.M + (size_t)x * NE;
     p2_load_tw<T, NY, NZ>(twz, twy, a, tid, NT);
-    const cpx<T>* qb = a.qs + ((size_t)slot * (size_t)a.qpitch + (size_t)ib) * (size_t)a.M + (size_t)x * NE;
-    const cpx<T>* qa = a.qs + ((size_t)slot * (size_t)a.qpitch + (size_t)ia) * (size_t)a.M + (size_t)x * NE;
-    const T* ef = spec == 0 ? a.e4A : a.e4B;
-    cpx<T> v[IZ1][A2], h[IZ1][A2];
+    const cpx<T>* q0 = a.qs + ((size_t)slot * (size_t)a.qpitch + (size_t)ib) * (size_t)a.M + (size_t)x * NE;
+    const cpx<T>* q1 = a.qs + ((size_t)slot * (size_t)a.qpitch + (size_t)ia) * (size_t)a.M + (size_t)x * NE;
+    for (int term = 0; term < (specB >= 0 ? 2 : 1); ++term) {
+        // term 0: b from entry ib, a from entry ia; term 1 the other way round
+        const cpx<T>* qb = term ? q1 : q0;
+        const cpx<T>* qa = term ? q0 : q1;
+        const T* ef = (term ? specB : specA) == 0 ? a.e4A : a.e4B;
+        const size_t out = fo + (term ? (size_t)a.nb * (size_t)a.M : 0);
+        cpx<T> v[IZ1][A2], h[IZ1][A2];
 #pragma unroll
-    for (int it = 0; it < IZ1; ++it) {
-        const int w = tid + it * NT;
+        for (int it = 0; it < IZ1; ++it) {
+            const int w = tid + it * NT;
 #pragma unroll
-        for (int a2 = 0; a2 < A2; ++a2) {
-            if (P::z1_valid(w)) {
-                const int e = P::z1_elem(w, a2);
-                const cpx<T> vb = qb[e], va = qa[e];
-                const T zb = pbk ? vb.y : vb.x, za = pak ? va.y : va.x, ev = ef[fo + e];
-                v[it][a2] = {Boltz<T>::mul4(zb, ev), Boltz<T>::mul4(za, ev)};
-                h[it][a2] = {Boltz<T>::mul2(zb, ev), Boltz<T>::mul2(za, ev)};
-            } else {
-                v[it][a2] = h[it][a2] = {T(0), T(0)};
+            for (int a2 = 0; a2 < A2; ++a2) {
+                if (P::z1_valid(w)) {
+                    const int e = P::z1_elem(w, a2);
+                    const cpx<T> vb = qb[e], va = qa[e];
+                    const T zb = pbk ? vb.y : vb.x, za = pak ? va.y : va.x, ev = ef[fo + e];
+                    v[it][a2] = {Boltz<T>::mul4(zb, ev), Boltz<T>::mul4(za, ev)};
+                    h[it][a2] = {Boltz<T>::mul2(zb, ev), Boltz<T>::mul2(za, ev)};
+                } else {
+                    v[it][a2] = h[it][a2] = {T(0), T(0)};
+                }
             }
         }
+        __syncthreads();  // twiddles; the previous term's last reads of buf
+        P::fwd(v, buf, twz, twy, a.Ch + out, tid);
+        __syncthreads();
+        P::fwd(h, buf, twz, twy, a.Cf + out, tid);
     }
-    __syncthreads();  // twiddles
-    P::fwd(v, buf, twz, twy, a.Ch + fo, tid);
-    __syncthreads();
-    P::fwd(h, buf, twz, twy, a.Cf + fo, tid);
 }
 
 // plane_mid on the two-stage FFT: Ch <- F2( E1/2 * F2^-1(Ch) )

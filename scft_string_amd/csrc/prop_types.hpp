@@ -212,4 +212,11 @@ struct Kinetic<float> {
     }
 };
 
+// Box-derivative constants of the gradient kernels: ds, 2 pi / D_i and D_i.
+struct GradConst {
+    double ds;
+    double kD[3];  // 2 pi / D_i
+    double D[3];
+};
+
 }  // namespace scft
