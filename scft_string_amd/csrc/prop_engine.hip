@@ -5,6 +5,7 @@
 #include "prop_pencil.hpp"
 #include "prop_sandwich.hpp"
 #include "prop_lines2.hpp"
+#include "prop_rlines.hpp"
 #include "grad_kernels.hpp"
 
 #include <algorithm>
@@ -519,12 +520,38 @@ static hipError_t lines_setup()
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         if (e != hipSuccess) return e;
     }
-    return hipSuccess;
+    hipError_t e = hipSuccess;
+#define RL_ATTR(L)                                                                                               \
+    static_assert(C2Geom<L, rl_ty<T>()>::ok && C2Geom<L, rl_tz<T>()>::ok, "register line pass " #L);        \
+    static_assert(rl_lds_bytes<T, L, rl_ty<T>()>() <= 160 * 1024, "k_ry LDS " #L);                            \
+    static_assert(rl_lds_bytes<T, L, rl_tz<T>()>() <= 160 * 1024, "k_rz LDS " #L);                            \
+    if (e == hipSuccess)                                                                                          \
+        e = hipFuncSetAttribute((const void*)k_ry<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,              \
+                                (int)rl_lds_bytes<T, L, rl_ty<T>()>());                                           \
+    if (e == hipSuccess)                                                                                          \
+        e = hipFuncSetAttribute((const void*)k_rz<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,              \
+                                (int)rl_lds_bytes<T, L, rl_tz<T>()>());
+    RL_LENGTHS(RL_ATTR)
+#undef RL_ATTR
+    return e;
 }
 
 template <typename T>
 static void lines_ly(const PropArgs<T>& a, int nh, int nfields, const PencilTiles& p, size_t lds, hipStream_t st)
 {
+    if (a.rlines) switch (a.m1) {
+#define RL_CASE(L)                                                                                                \
+    case L: {                                                                                                     \
+        constexpr int TY = rl_ty<T>();                                                                            \
+        using RG = C2Geom<L, TY>;                                                                                 \
+        hipLaunchKernelGGL((k_ry<T, L>), dim3(a.m0 * ((a.m2 + TY - 1) / TY), nfields), dim3(RG::NT),             \
+                           (rl_lds_bytes<T, L, TY>()), st, a, nh);                                                \
+        return;                                                                                                   \
+    }
+            RL_LENGTHS(RL_CASE)
+#undef RL_CASE
+            default: break;
+        }
     const int ntz = (a.m2 + p.ty - 1) / p.ty;
     const dim3 g(a.m0 * ntz, nfields);
     if (a.pencil_ct) switch (a.m1) {
@@ -541,6 +568,19 @@ template <typename T>
 static void lines_lz(const PropArgs<T>& a, int s, const PencilTiles& p, size_t lds, hipStream_t st)
 {
     const long long lines = (long long)a.m0 * a.m1;
+    if (a.rlines) switch (a.m2) {
+#define RL_CASE(L)                                                                                                \
+    case L: {                                                                                                     \
+        constexpr int TZ = rl_tz<T>();                                                                            \
+        using RG = C2Geom<L, TZ>;                                                                                 \
+        hipLaunchKernelGGL((k_rz<T, L>), dim3((unsigned)((lines + TZ - 1) / TZ), a.nb), dim3(RG::NT),            \
+                           (rl_lds_bytes<T, L, TZ>()), st, a, s);                                                 \
+        return;                                                                                                   \
+    }
+            RL_LENGTHS(RL_CASE)
+#undef RL_CASE
+            default: break;
+        }
     const dim3 g((unsigned)((lines + p.nlz - 1) / p.nlz), a.nb);
     if (a.pencil_ct) switch (a.m2) {
 #define PC_CASE(L) \
@@ -620,6 +660,8 @@ std::string PropEngine<T>::init(const int m[3], const double D[3], int N, int nb
     const scft_schedule sc = sched ? *sched : scft_schedule{0, 0, 0, 0};
     const char* pen_rt = getenv("SCFT_PENCIL_RT");  // =1: runtime-length line kernels only
     pencil_ct_ = !(pen_rt && pen_rt[0] == '1');
+    const char* rl = getenv("SCFT_RLINES");  // =0: LDS Stockham y and z passes in the split step
+    rlines_ = pencil_ct_ && !(rl && rl[0] == '0');
     const KernelSet<T>* kset = find_kernels<T>(m[0], m[1], m[2]);
     const bool pencil_ok = pencil_supported(m[0], m[1], m[2]);
     // persistent plane_mid (grids whose kernel set has it): one workgroup per CU;
@@ -924,6 +966,7 @@ PropArgs<T> PropEngine<T>::args(int nb, const ChainConst& cc) const
     a.Nh = cc.Nh;
     a.nb = nb;
     a.pencil_ct = pencil_ct_ ? 1 : 0;
+    a.rlines = rlines_ ? 1 : 0;
     a.coltile = coltile_;
     a.pgrid = pgrid_;
     a.planent = planent_;

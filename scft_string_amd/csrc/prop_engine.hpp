@@ -140,6 +140,7 @@ private:
     void clear_graphs();
     bool use_graph_ = true;
     bool pencil_ct_ = true;  // line kernels compiled for the axis length where one exists
+    bool rlines_ = true;     // split line step: register-FFT y and z passes (prop_rlines.hpp)
     int m_[3] = {0, 0, 0};
     double D_[3] = {0, 0, 0};
     long long M_ = 0;

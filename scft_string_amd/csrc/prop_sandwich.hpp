@@ -151,7 +151,10 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, P2Geom<NY, NZ>::NT
 // FP64 residual of its outputs' sum against the FP64 line sum / N0 into one output
 // (inv_sum_fix), as every stage-1' item does against its own DC input. The line sum is then
 // exact to the rounding of single outputs, which no longer lines up across lines.
-template <typename G, typename T, int N0, int TC>
+//
+// ROW: stage-1 item w is (c, n1) = (w / X1, w % X1) instead of (w % TC, w / TC), so that
+// consecutive lanes hold consecutive elements of one line (lines contiguous in memory).
+template <typename G, typename T, int N0, int TC, bool ROW = false>
 __device__ __forceinline__ void sc_x_sandwich(cpx<T> (&v)[G::I1][G::X2], cpx<T>* buf, const cpx<T>* tw0, const T* gx,
                                               bool full, const InvLen<T>& inv, double2* dsum, int tid)
 {
@@ -161,7 +164,7 @@ __device__ __forceinline__ void sc_x_sandwich(cpx<T> (&v)[G::I1][G::X2], cpx<T>*
     for (int it = 0; it < I1; ++it) {
         const int w = tid + it * NT;
         if ((TC * X1) % NT == 0 || w < TC * X1) {
-            const int c = w % TC, n1 = w / TC;
+            const int c = ROW ? w / X1 : w % TC, n1 = ROW ? w % X1 : w / TC;
             if constexpr (F32) {
                 double sx = 0.0, sy = 0.0;
 #pragma unroll
@@ -233,7 +236,7 @@ __device__ __forceinline__ void sc_x_sandwich(cpx<T> (&v)[G::I1][G::X2], cpx<T>*
     for (int it = 0; it < I1; ++it) {
         const int w = tid + it * NT;
         if ((TC * X1) % NT == 0 || w < TC * X1) {
-            const int c = w % TC, n1 = w / TC;
+            const int c = ROW ? w / X1 : w % TC, n1 = ROW ? w % X1 : w / TC;
             sfor<X2>([&](auto kc) {
                 constexpr int k2 = decltype(kc)::value;
                 v[it][k2] = buf[c * CX + n1 * PX + k2];

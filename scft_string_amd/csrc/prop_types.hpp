@@ -14,6 +14,7 @@ struct PropArgs {
     int N, n1, n2, Nh;     // contour steps, block split, homopolymer length
     int nb;                // active batch slots
     int pencil_ct;         // line kernels: use the compile-time-length ones where they exist
+    int rlines;            // split line step: register-FFT y and z passes where compiled
     int coltile;           // two-stage column pass: tile width (0: the grid's default)
     int pgrid;             // planes with a persistent plane_mid: > 0 runs it on that many
                            // workgroups; 0 launches one workgroup per plane
