@@ -128,11 +128,15 @@ __host__ __device__ __forceinline__ void rdft(cpx<T>* v)
 }
 
 // Two-stage split N = N1 * N2 with N1 <= N2 <= 16 (N1 = 1 when N <= 16).
+// N2 up to 25 (a 20- or 25-point register DFT) only when no split with N2 <= 16 exists, so
+// that lengths such as 200 = 10 x 20 split too.
 __host__ __device__ constexpr int split_n1(int N)
 {
     if (N <= 16) return fft_len_supported(N) ? 1 : 0;
-    for (int d = 16; d >= 2; --d)
-        if (N % d == 0 && d * d <= N && N / d <= 16 && fft_len_supported(d) && fft_len_supported(N / d)) return d;
+    for (int lim = 16; lim <= 25; lim += 9)
+        for (int d = 16; d >= 2; --d)
+            if (N % d == 0 && d * d <= N && N / d <= lim && fft_len_supported(d) && fft_len_supported(N / d))
+                return d;
     return 0;
 }
 

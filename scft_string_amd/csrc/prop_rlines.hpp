@@ -4,6 +4,8 @@
 //
 //   k_ry  (h, f):  h <- K1/2y h,  f <- Ky f           (or h alone)
 //   k_rz  (h, f):  h <- K1/2z E1/2 K1/2z h,  f <- Kz f   (E1/2 of step s)
+// and the x pass is the column pass k_scol2 of the two-pass step (prop_sandwich.hpp), which
+// completes the same step from the same field state.
 //
 // The same one-axis sandwiches as k_ly and k_lz (density.h:129-220, GPU_kernels.h:207-213,
 // the kinetic factor split by axis), on the same real-space natural-order fields, so either
@@ -28,6 +30,16 @@ constexpr int rl_ty()
 }
 template <typename T>
 constexpr int rl_tz()
+{
+    return sizeof(T) == 8 ? 8 : 16;
+}
+
+// The x pass of the split step runs the column pass k_scol2 with the (y, z) extent given at
+// run time (NY = NZ = 0), for these x lengths (200 = 10 x 20: a 20-point stage-1 DFT), on
+// rl_tx<T>() columns per workgroup.
+#define RX_LENGTHS(X) X(96) X(128) X(150) X(160) X(192) X(200) X(256)
+template <typename T>
+constexpr int rl_tx()
 {
     return sizeof(T) == 8 ? 8 : 16;
 }

@@ -262,13 +262,20 @@ constexpr size_t scol_lds_bytes()
 // h <- K1/2x h, q(s+1) = (4 E1/4 h - E1/2 f) / 3 to the q store; PL_BEGIN starts the contour
 // from q(0) = 1, PL_RESUME from the stored q at index s (entry a.qsrc). Then, except for
 // PL_LAST, the fields of the next step: Ch <- K1/2x E1/4 q, Cf <- Kx E1/2 q.
+// NY = NZ = 0: any (y, z) extent, NC = m1 m2 at run time (the x pass of the split line step,
+// whose fields hold the same state between passes).
 template <typename T, int N0, int NY, int NZ, int TC>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<N0, TC>::NT)))
 k_scol2(PropArgs<T> a, int s, int mode)
 {
     using G = C2Geom<N0, TC>;
     constexpr int NT = G::NT, X1 = G::X1, X2 = G::X2, I1 = G::I1;
-    constexpr int NC = NY * NZ;
+    const int NC = NY > 0 ? NY * NZ : a.m1 * a.m2;
+    constexpr bool ctile = NY > 0 && (NY * NZ) % TC == 0;  // every tile full
+    // f is read before the h sandwich, which its reads overlap, where a line's f fits in
+    // registers beside h (X2 elements of 8 bytes or fewer, or a compiled plane grid); longer
+    // lines read it after the sandwich
+    constexpr bool early_f = NY > 0 || X2 * sizeof(T) <= 64;
     extern __shared__ unsigned char smem_raw[];
     cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
     cpx<T>* tw0 = buf + G::LDSN;
@@ -285,7 +292,7 @@ k_scol2(PropArgs<T> a, int s, int mode)
     }
     const InvLen<T> inv(N0);
     // item it of this thread: valid, and its column inside the grid
-    auto item_ok = [&](int w) { return ((TC * X1) % NT == 0 || w < TC * X1) && (NC % TC == 0 || c0 + w % TC < NC); };
+    auto item_ok = [&](int w) { return ((TC * X1) % NT == 0 || w < TC * X1) && (ctile || c0 + w % TC < NC); };
     auto elem = [&](int w, int n2) { return (size_t)(w / TC + X1 * n2) * NC + (size_t)(c0 + w % TC); };
     auto load = [&](const cpx<T>* p, cpx<T> (&v)[I1][X2]) {
 #pragma unroll
@@ -335,10 +342,11 @@ k_scol2(PropArgs<T> a, int s, int mode)
         __syncthreads();  // tables
     } else {
         load(a.Ch + so, q);  // q holds h until the combination below
-        cpx<T> f[I1][X2];    // f is read before the h sandwich, which its reads overlap
-        load(a.Cf + so, f);
-        __syncthreads();     // tables
+        cpx<T> f[I1][X2];
+        if (early_f) load(a.Cf + so, f);
+        __syncthreads();  // tables
         sc_x_sandwich<G, T, N0, TC>(q, buf, tw0, gx, false, inv, dsum, tid);
+        if (!early_f) load(a.Cf + so, f);
         load_factors();
         const bool qa = s < a.n1, pb = s < a.n2;  // species of step s (density.h:136-146)
 #pragma unroll

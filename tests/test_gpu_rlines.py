@@ -1,9 +1,10 @@
-"""The split line step's register-FFT y and z passes (k_ry, k_rz: the two-stage register
-FFT of the column pass along y and z, for the axis lengths 96, 128, 150, 160, 192, 256)
-against the FP64 oracle and against the LDS Stockham passes (SCFT_RLINES=0) on the same
-fields, through the C-ABI. Grids cover partial z tiles of the y pass (m2 = 150), partial
-row tiles of the z pass (m0 m1 not a multiple of 32), and grids where only one of the two
-axes has a register pass."""
+"""The split line step's register-FFT passes against the FP64 oracle and against the LDS
+Stockham passes (SCFT_RLINES=0) on the same fields, through the C-ABI: the y and z passes
+(k_ry, k_rz: the two-stage register FFT of the column pass along y and z, axis lengths 96,
+128, 150, 160, 192, 256) and, in FP32, the x pass (the column kernel k_scol2 with the grid's
+(y, z) extent at run time, x lengths 96 ... 256 and 200). Grids cover partial z tiles of the
+y pass (m2 = 150), partial row tiles of the z pass (m0 m1 not a multiple of the tile), and
+grids where only some axes have a register pass."""
 import contextlib
 import os
 import sys
@@ -114,3 +115,31 @@ def test_register_line_passes_fp32_lamella(axis):
     dphi = float(np.max(np.abs(out["fp32"][0] - out["fp64"][0])))
     _say(f"lamella varying along axis {other}: fp32 vs fp64 |dlnQ| {dl:.2e}, max |dphi| {dphi:.2e}")
     assert dl <= 3e-6 and dphi <= 1e-5
+
+
+@pytest.mark.parametrize("m", [(200, 30, 24), (128, 20, 150)], ids=lambda m: "x".join(map(str, m)))
+def test_register_x_pass_fp32(m):
+    """FP32: the x pass runs the two-pass step's column kernel with the grid's (y, z) extent
+    at run time (x lengths 96 ... 256 and 200 = 10 x 20). Against the Stockham passes in FP32,
+    and against FP64 on a lamella along x (every x line identical, so FP32 line-sum roundings
+    would add up coherently without the FP64 line sums)."""
+    D = tuple(0.08 * v for v in m)
+    N = 24
+    W = np.stack([_fields(m, D, seed=80 + i) for i in range(2)])
+    with Context(m, D, N, 0.45, 0.1, 1, 0.8, chi=15.0, nrep=2, precision="fp32", schedule=SPLIT) as c1:
+        phi1, lnQ1, _ = c1.props_batch(W)
+    with _stockham_yz():
+        with Context(m, D, N, 0.45, 0.1, 1, 0.8, chi=15.0, nrep=2, precision="fp32", schedule=SPLIT) as c0:
+            phi0, lnQ0, _ = c0.props_batch(W)
+    dl, dphi = float(np.max(np.abs(lnQ1 - lnQ0))), _rel(phi1, phi0)
+    M = int(np.prod(m))
+    x = np.meshgrid(*[np.arange(m[i]) * D[i] / m[i] for i in range(3)], indexing="ij")[0]
+    Wl = np.stack([1.5 * np.cos(2 * np.pi * x / D[0]).reshape(-1), np.zeros(M)])[None]
+    out = {}
+    for prec in ("fp64", "fp32"):
+        with Context(m, D, 40, 0.5, 0.1, 1, 0.8, chi=15.0, precision=prec, schedule=SPLIT) as ctx:
+            out[prec] = ctx.props_batch(Wl)
+    dll = abs(float(out["fp32"][1][0] - out["fp64"][1][0]))
+    _say(f"x pass fp32 {m}: vs Stockham |dlnQ| {dl:.2e} rel dphi {dphi:.2e}; x-lamella vs fp64 |dlnQ| {dll:.2e}")
+    assert dl <= 3e-6 * max(1.0, float(np.max(np.abs(lnQ0)))) and dphi <= 3e-6
+    assert dll <= 3e-6
