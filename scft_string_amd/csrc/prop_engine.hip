@@ -533,15 +533,17 @@ static hipError_t lines_setup()
                                 (int)rl_lds_bytes<T, L, rl_tz<T>()>());
     RL_LENGTHS(RL_ATTR)
 #undef RL_ATTR
-    if constexpr (std::is_same<T, float>::value) {
 #define RX_ATTR(L)                                                                                               \
-    static_assert(C2Geom<L, rl_tx<T>()>::ok && scol_lds_bytes<T, L, rl_tx<T>()>() <= 160 * 1024, "x pass " #L); \
-    if (e == hipSuccess)                                                                                          \
-        e = hipFuncSetAttribute((const void*)k_scol2<T, L, 0, 0, rl_tx<T>()>,                                     \
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)scol_lds_bytes<T, L, rl_tx<T>()>());
-        RX_LENGTHS(RX_ATTR)
-#undef RX_ATTR
+    if constexpr (rx_ok<T>(L)) {                                                                                  \
+        static_assert(C2Geom<L, rl_tx<T>(), rl_x1(L)>::ok, "x pass " #L);                                      \
+        static_assert(scol_lds_bytes<T, L, rl_tx<T>(), rl_x1(L)>() <= 160 * 1024, "x pass LDS " #L);          \
+        if (e == hipSuccess)                                                                                      \
+            e = hipFuncSetAttribute((const void*)k_scol2<T, L, 0, 0, rl_tx<T>(), rl_x1(L)>,                       \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,                                   \
+                                    (int)scol_lds_bytes<T, L, rl_tx<T>(), rl_x1(L)>());                           \
     }
+    RX_LENGTHS(RX_ATTR)
+#undef RX_ATTR
     return e;
 }
 
@@ -604,20 +606,20 @@ static void lines_lz(const PropArgs<T>& a, int s, const PencilTiles& p, size_t l
 template <typename T>
 static void lines_lx(const PropArgs<T>& a, int s, int mode, const PencilTiles& p, size_t lds, hipStream_t st)
 {
-    // FP32 only: in FP64 the 20-point stage-1 items of q, f and both factor rows need 256
-    // VGPRs (one wave per SIMD), and the pass ran slower than k_lx on the shipped box
-    if constexpr (std::is_same<T, float>::value) {
+    if (a.rlines) {
         const int pm = mode == PZ_BEGIN ? PL_BEGIN : mode == PZ_RESUME ? PL_RESUME : mode == PZ_LAST ? PL_LAST : PL_STEP;
         const int nc = a.m1 * a.m2;
-        if (a.rlines) switch (a.m0) {
+        switch (a.m0) {
 #define RX_CASE(L)                                                                                                \
-    case L: {                                                                                                     \
-        constexpr int TX = rl_tx<T>();                                                                            \
-        using RG = C2Geom<L, TX>;                                                                                 \
-        hipLaunchKernelGGL((k_scol2<T, L, 0, 0, TX>), dim3((nc + TX - 1) / TX, a.nb), dim3(RG::NT),              \
-                           (scol_lds_bytes<T, L, TX>()), st, a, s, pm);                                           \
-        return;                                                                                                   \
-    }
+    case L:                                                                                                       \
+        if constexpr (rx_ok<T>(L)) {                                                                              \
+            constexpr int TX = rl_tx<T>(), X1 = rl_x1(L);                                                         \
+            using RG = C2Geom<L, TX, X1>;                                                                         \
+            hipLaunchKernelGGL((k_scol2<T, L, 0, 0, TX, X1>), dim3((nc + TX - 1) / TX, a.nb), dim3(RG::NT),      \
+                               (scol_lds_bytes<T, L, TX, X1>()), st, a, s, pm);                                   \
+            return;                                                                                               \
+        }                                                                                                         \
+        break;
             RX_LENGTHS(RX_CASE)
 #undef RX_CASE
             default: break;

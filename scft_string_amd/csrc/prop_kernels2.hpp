@@ -747,9 +747,12 @@ k_plane_mid2p(PropArgs<T> a, int s, int nplanes)
 
 // Column pass on the two-stage FFT: for TC consecutive (y,z) columns of one field, forward
 // FFT along x, multiply by the kinetic factor, inverse FFT along x (see k_column).
-template <int N0, int TC>
+// X1O > 0 sets the split N0 = X1 X2 (X1 = X1O): stage-1 items hold X2 elements, stage-2
+// items X1, so a larger X1 moves the longer register DFT to the stage where fewer fields
+// are live.
+template <int N0, int TC, int X1O = 0>
 struct C2Geom {
-    static constexpr int X1 = split_n1(N0), X2 = N0 / X1;
+    static constexpr int X1 = X1O > 0 ? X1O : split_n1(N0), X2 = N0 / X1;
     static constexpr int NT = cdiv(TC * X1, 64) * 64 > 1024 ? 1024 : cdiv(TC * X1, 64) * 64;
     static constexpr int I1 = cdiv(TC * X1, NT), I2 = cdiv(TC * X2, NT);
     static constexpr int PX = X2, CX = X1 * PX + 1;  // TX: c*CX + n1*PX + k2

@@ -35,13 +35,25 @@ constexpr int rl_tz()
 }
 
 // The x pass of the split step runs the column pass k_scol2 with the (y, z) extent given at
-// run time (NY = NZ = 0), for these x lengths (200 = 10 x 20: a 20-point stage-1 DFT), on
-// rl_tx<T>() columns per workgroup.
+// run time (NY = NZ = 0), for these x lengths, on rl_tx<T>() columns per workgroup (256
+// bytes of each x row: 128 was slower), except FP64 at 256, which needs 256 VGPRs.
 #define RX_LENGTHS(X) X(96) X(128) X(150) X(160) X(192) X(200) X(256)
 template <typename T>
 constexpr int rl_tx()
 {
-    return sizeof(T) == 8 ? 8 : 16;
+    return sizeof(T) == 8 ? 16 : 32;
+}
+template <typename T>
+constexpr bool rx_ok(int L)
+{
+    return !(sizeof(T) == 8 && L == 256);
+}
+// Split of the x pass: the larger factor first (200 = 20 x 10, 128 = 16 x 8), so that the
+// stage-1 items, which hold q, f and the factor rows, take the shorter DFT and the longer
+// one runs in stage 2 on one field at a time
+constexpr int rl_x1(int L)
+{
+    return L / split_n1(L);
 }
 
 template <typename T, int L, int TC>

@@ -250,10 +250,10 @@ __device__ __forceinline__ void sc_x_sandwich(cpx<T> (&v)[G::I1][G::X2], cpx<T>*
 
 // Dynamic LDS of k_scol2: the transpose buffer, twiddles and kinetic table of the x axis,
 // and in FP32 the FP64 item sums of sc_x_sandwich.
-template <typename T, int N0, int TC>
+template <typename T, int N0, int TC, int X1O = 0>
 constexpr size_t scol_lds_bytes()
 {
-    using G = C2Geom<N0, TC>;
+    using G = C2Geom<N0, TC, X1O>;
     const size_t base = (size_t)(G::LDSN + N0) * sizeof(cpx<T>) + (size_t)N0 * sizeof(T);
     return std::is_same<T, float>::value ? (base + 15) / 16 * 16 + (size_t)TC * G::X1 * 16 : base;
 }
@@ -264,11 +264,11 @@ constexpr size_t scol_lds_bytes()
 // PL_LAST, the fields of the next step: Ch <- K1/2x E1/4 q, Cf <- Kx E1/2 q.
 // NY = NZ = 0: any (y, z) extent, NC = m1 m2 at run time (the x pass of the split line step,
 // whose fields hold the same state between passes).
-template <typename T, int N0, int NY, int NZ, int TC>
-__global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<N0, TC>::NT)))
+template <typename T, int N0, int NY, int NZ, int TC, int X1O = 0>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<N0, TC, X1O>::NT)))
 k_scol2(PropArgs<T> a, int s, int mode)
 {
-    using G = C2Geom<N0, TC>;
+    using G = C2Geom<N0, TC, X1O>;
     constexpr int NT = G::NT, X1 = G::X1, X2 = G::X2, I1 = G::I1;
     const int NC = NY > 0 ? NY * NZ : a.m1 * a.m2;
     constexpr bool ctile = NY > 0 && (NY * NZ) % TC == 0;  // every tile full
