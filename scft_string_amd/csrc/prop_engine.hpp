@@ -41,9 +41,9 @@ struct KernelSet {
                    cpx<T>* Gf, cpx<T>* K0, cpx<T>* Kf, const GradConst& gc, double* part, hipStream_t);
     // two-stage plane kernels: the default workgroup size and the alternative (0: none)
     int plane_nt, plane_nt_alt;
-    // the two-pass step (prop_sandwich.hpp; nullptr when the grid has none) and its other
-    // column tile width (0: none)
-    int tc_alt3;
+    // the two-pass step (prop_sandwich.hpp; nullptr when the grid has none), its default
+    // column tile width and the other one (0: none)
+    int tc3, tc_alt3;
     hipError_t (*setup3)();
     void (*begin3)(const PropArgs<T>&, hipStream_t);
     void (*step3)(const PropArgs<T>&, int s, hipStream_t);
