@@ -23,7 +23,7 @@
 // k^2 = sum_j (2 pi n_j / D_j)^2 (density.h:79-95). That is the line pipeline (any 2-3-5
 // grid). Grids with a two-stage kernel set take four passes per step instead:
 //   k_gplane_init2   Z0, Zf from the q store and E4, E2 -> (y, z) spectra   (A, B)
-//   k_gcol2(SPLIT)   A: forward x -> K0, times K12, inverse x -> A;  B: forward x -> Kf
+//   k_gcol2          A: forward x -> K0, times K12, inverse x -> A;  B: forward x -> Kf
 //   k_plane_mid2     A: inverse (y, z), times E2, forward (y, z)
 //   k_gcol2_acc      A: forward x -> K1 in registers, the sum over k with K0, Kf read at +-k
 #pragma once
@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(RED_NT) k_gacc(const cpx<T>* __restrict__ K0, 
 // Last pass of a box-gradient term on the two-stage FFT, fused with the reduction over k.
 // Ch of each slot holds the (y, z) spectrum of Z1 = E1/2 K12 Z0 with x in natural order; the
 // forward FFT along x gives K1(k) on the workgroup's TC columns, which is paired in registers
-// with K0(-k), Kf(-k) and Kf(k) (written by k_gcol2 GC_SPLIT) to add, per slot and
+// with K0(-k), Kf(-k) and Kf(k) (written by k_gcol2) to add, per slot and
 // direction i, the columns' share of
 //   sum_k [4/3 dK12_i(k) Im(K0(-k) K1(k)) - 1/6 dK6_i(k) Im(Kf(-k) Kf(k))]
 // to part[slot][tile][i] (scaled by M^-2 on the host), the sum k_gacc forms from stored

@@ -269,7 +269,7 @@ struct KS {
             hipLaunchKernelGGL((k_gplane_init2<T, N1, N2>), dim3(N0, nb), dim3(PG::NT), lds2_plane, st, w, ib, pbk, ia,
                                pak, specA, specB);
             hipLaunchKernelGGL((k_gcol2<T, N0, N1, N2, TC>), dim3(ntile, 2 * nt), dim3(CG::NT), lds2_col, st, w, nt,
-                               (int)GC_SPLIT, K0, Kf);
+                               K0, Kf);
             for (int t = 0; t < nterm; ++t) {
                 PropArgs<T> b = a;  // k_plane_mid2 takes the real part's species from s < n1, the
                 b.Ch = Gh + (size_t)t * nb * a.M;  // imaginary part's from s < n2: at s = 0 both

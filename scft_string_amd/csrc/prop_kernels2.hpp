@@ -912,13 +912,11 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<N0, TC>::NT
 
 // Column pass of a box-gradient term on the two-stage FFT. Field f < nh is Ch of slot f,
 // f >= nh is Cf of slot f - nh. Every field: forward FFT along x, the spectrum stored to
-// K0 (f < nh) or Kf (f >= nh) in natural (kx, ky, kz) order. GC_SPLIT, f < nh: then times
-// the half-step kinetic factor / M and inverse along x, back to Ch (as k_column2).
-enum GColMode { GC_SPLIT = 0, GC_FWD = 1 };
-
+// K0 (f < nh) or Kf (f >= nh) in natural (kx, ky, kz) order; f < nh: then times the
+// half-step kinetic factor / M and inverse along x, back to Ch (as k_column2).
 template <typename T, int N0, int NY, int NZ, int TC>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<N0, TC>::NT)))
-k_gcol2(PropArgs<T> a, int nh, int mode, cpx<T>* __restrict__ K0, cpx<T>* __restrict__ Kf)
+k_gcol2(PropArgs<T> a, int nh, cpx<T>* __restrict__ K0, cpx<T>* __restrict__ Kf)
 {
     using G = C2Geom<N0, TC>;
     constexpr int NT = G::NT, X1 = G::X1, X2 = G::X2, I1 = G::I1, I2 = G::I2, CX = G::CX, PX = G::PX;
@@ -965,7 +963,7 @@ k_gcol2(PropArgs<T> a, int nh, int mode, cpx<T>* __restrict__ K0, cpx<T>* __rest
         }
     }
     __syncthreads();
-    const bool back = mode == GC_SPLIT && !isf;  // uniform over the workgroup
+    const bool back = !isf;  // uniform over the workgroup
     cpx<T> u[I2][X1];
 #pragma unroll
     for (int it = 0; it < I2; ++it) {
