@@ -169,14 +169,15 @@ def _lamellae(m, D):
 
 
 def test_saddles_converged_independently_in_fp32_and_fp64():
-    """Three saddle points (x-, y- and z-lamellae in a 32^3 box of side 4.5, chiN = 15,
-    f = 0.5, phidb = 0.8, N = 40, canonical) converged separately in FP32 and in FP64 from the
+    """Three saddle points (x-, y- and z-lamellae in a 32^3 box of side 1.5, one lamellar
+    period: chiN = 15, f = 0.5, phidb = 0.8, N = 40, canonical; a box of side 4.5 relaxes a
+    one-period cosine to the disordered state) converged separately in FP32 and in FP64 from the
     same start, as independent replicas with Anderson mixing (H = 20) until errs stops
     falling. FreeE of each: with F from an FP64 props on the FP32-converged fields (fe_fp64)
     |dF| <= 1e-6 per replica and between replicas, since F is stationary at the saddle and
     the FP32 field error enters at second order; the all-FP32 F carries the FP32 error of lnQ
     at first order and is bounded by 1e-6 too (measured 6.6e-7)."""
-    m, D, N = (32, 32, 32), (4.5, 4.5, 4.5), 40
+    m, D, N = (32, 32, 32), (1.5, 1.5, 1.5), 40
     args = (m, D, N, 0.5, 0.1, 1, 0.8, 15.0)
     W = _lamellae(m, D)
     fe, errs = {}, {}
@@ -195,6 +196,7 @@ def test_saddles_converged_independently_in_fp32_and_fp64():
             errs[prec] = (best, k + 1)
             Wc = ctx.get_fields()
             fe[prec] = ctx.free_energy()["FEs"]
+            amp = np.ptp(Wc.reshape(3, 2, -1)[:, 0], axis=1)
         if prec == "fp32":
             with Context(*args, nrep=3, precision=prec, dostring=0, fe_fp64=0) as ctx:
                 ctx.set_fields(Wc)
@@ -206,6 +208,7 @@ def test_saddles_converged_independently_in_fp32_and_fp64():
          f"between {np.abs(d_all - d_all[0]).max():.1e}")
     assert errs["fp64"][0] < 1e-8 and errs["fp32"][0] < 1e-4
     assert np.ptp(fe["fp64"]) < 1e-9                      # three symmetric saddles
+    assert np.all(fe["fp64"] < 3.55) and np.all(amp > 5.0)  # ordered (disordered: F = 3.6, flat W-)
     assert np.abs(d).max() <= 1e-6 and np.abs(d - d[0]).max() <= 1e-6
     assert np.abs(d_all).max() <= 1e-6 and np.abs(d_all - d_all[0]).max() <= 1e-6
 
@@ -234,3 +237,52 @@ def test_fp32_props_of_a_lamella_along_each_axis_within_1e6(axis):
     dl = abs(res["fp32"][0] - res["fp64"][0])
     _say(f"lamella along axis {axis}: all-FP32 |dF| {dF:.2e}, |dlnQ| {dl:.2e} (lnQ {res['fp64'][0]:.4f})")
     assert dF <= 1e-6 and dl <= 3e-6
+
+
+def test_string_converged_independently_in_fp32_and_fp64():
+    """A 5-replica string between an x- and a y-lamella (16^3 box of side 1.5, one lamellar
+    period; chiN = 15, f = 0.5, phidb = 0.8, N = 40, canonical), both ends fixed at the
+    FP64-converged lamellae, converged separately in FP32 and in FP64 from the same linear
+    interpolation for 1500 iterations: FP64 errs3 falls to ~1e-14, FP32 to its rounding floor
+    (~2e-6). FreeE of the two strings: the FP32 context (final props in FP64, the default)
+    within 1e-6 of FP64 per replica and between replicas (measured 9e-8). The all-FP32 FreeE
+    (fe_fp64 = 0) carries the FP32 lnQ error of the interior replicas at first order: measured
+    0.6-1.2e-6 over runs, bounded here by 2e-6."""
+    m, Dl, N = (16, 16, 16), 1.5, 40
+    D = (Dl, Dl, Dl)
+    args = (m, D, N, 0.5, 0.1, 1, 0.8, 15.0)
+    M = int(np.prod(m))
+    x, y, _ = np.meshgrid(*[np.arange(m[i]) * D[i] / m[i] for i in range(3)], indexing="ij")
+    W = np.zeros((2, 2, M))
+    W[0, 0] = 2.0 * np.cos(2 * np.pi * x / Dl).reshape(-1)
+    W[1, 0] = 2.0 * np.cos(2 * np.pi * y / Dl).reshape(-1)
+    with Context(*args, nrep=2, precision="fp64", dostring=0, anderson=20) as ctx:
+        ctx.set_fields(W)
+        for k in range(1500):
+            if float(np.max(ctx.string_iterate(k, 1e-10)["errs"])) < 1e-11:
+                break
+        Wend = ctx.get_fields().reshape(2, 2, M)
+    R = 5
+    W0 = np.stack([(1 - t) * Wend[0] + t * Wend[1] for t in np.linspace(0, 1, R)])
+    fe, e3 = {}, {}
+    for prec in ("fp64", "fp32"):
+        with Context(*args, nrep=R, precision=prec, dostring=1, fix=(1, 1)) as ctx:
+            ctx.set_fields(W0)
+            for k in range(1500):
+                rep = ctx.string_iterate(k, 1e-6)
+            e3[prec] = float(np.max(rep["errs3"]))
+            fe[prec] = ctx.free_energy()["FEs"]
+            Wc = ctx.get_fields()
+        if prec == "fp32":
+            with Context(*args, nrep=R, precision=prec, dostring=1, fix=(1, 1), fe_fp64=0) as ctx:
+                ctx.set_fields(Wc)
+                fe["fp32_all"] = ctx.free_energy()["FEs"]
+    d = fe["fp32"] - fe["fp64"]
+    d_all = fe["fp32_all"] - fe["fp64"]
+    _say(f"string errs3 fp64 {e3['fp64']:.1e} fp32 {e3['fp32']:.1e}; FEs fp64 {fe['fp64']}; |dF| {np.abs(d).max():.1e} "
+         f"between {np.abs(d - d[0]).max():.1e}; all-FP32 |dF| {np.abs(d_all).max():.1e} "
+         f"between {np.abs(d_all - d_all[0]).max():.1e}")
+    assert e3["fp64"] < 1e-10 and e3["fp32"] < 1e-5
+    assert fe["fp64"][0] < 3.55 and np.all(fe["fp64"][1:-1] > fe["fp64"][0] + 1e-3)  # ordered ends, a barrier
+    assert np.abs(d).max() <= 1e-6 and np.abs(d - d[0]).max() <= 1e-6
+    assert np.abs(d_all).max() <= 2e-6 and np.abs(d_all - d_all[0]).max() <= 2e-6
