@@ -231,8 +231,9 @@ class Context:
 
     @property
     def fft_path(self):
-        """"two-stage" (or "two-stage-c32" / "two-stage-c64" at the other column tile width),
-        "stockham" or "pencil": the propagator's FFT kernels for this grid."""
+        """The propagator's contour-step kernels for this grid: "two-pass", "two-stage" (with
+        "-c32" / "-c64" appended at the family's other column tile width), "stockham",
+        "pencil-split" or "pencil"."""
         return lib().scft_fft_path(self._h).decode()
 
     @property

@@ -289,9 +289,11 @@ struct KS {
                        lds_plane > 80 * 1024, TC, alt_ok ? TC_ALT : 0, persist_ok, nullptr, nullptr};
         k.plane_nt = PG::NT;
         k.plane_nt_alt = palt_ok ? NT_ALT : 0;
-        // two-pass default tile: TC, but 32 at x = 128 (128^3 FP32, 8 replicas: 0.369 against
-        // 0.404 ms per step at one stream, 0.326 against 0.368 at two)
-        k.tc3 = (N0 == 128 && tp_alt_ok) ? TC_ALT : TC;
+        // two-pass default tile: 32 columns wherever that kernel exists. The autotuner picks it
+        // on every grid measured: 128^3 FP32 x 8, 0.369 against 0.404 ms per step at one
+        // stream; 64^3 FP32 x 1, 0.0250 against 0.0265; 64^3 FP64 x 1, 0.0340 against 0.0376;
+        // 32^3 FP32 x 1, 0.0166 against 0.0173; the 14-replica bench string at 3 streams
+        k.tc3 = tp_alt_ok ? 32 : TC;
         k.tc_alt3 = tp_alt_ok ? (k.tc3 == TC ? TC_ALT : TC) : 0;
         if (tp_ok) {
             k.setup3 = &setup_tp;

@@ -263,13 +263,14 @@ def test_two_stage_and_stockham_paths_agree(prec, monkeypatch):
     assert abs(out["0"][1][0] - out["1"][1][0]) <= tol * max(1.0, abs(out["1"][1][0]))
 
 
-@pytest.mark.parametrize("fam", ["two-stage", "two-pass"])
-@pytest.mark.parametrize("m,prec,width", [((64, 64, 64), "fp32", 32), ((48, 48, 48), "fp64", 64),
-                                          ((32, 32, 32), "fp64", 32)])
+@pytest.mark.parametrize("fam,m,prec,width", [
+    ("two-stage", (64, 64, 64), "fp32", 32), ("two-stage", (48, 48, 48), "fp64", 64),
+    ("two-stage", (32, 32, 32), "fp64", 32), ("two-pass", (64, 64, 64), "fp32", 64),
+    ("two-pass", (48, 48, 48), "fp64", 64), ("two-pass", (32, 32, 32), "fp64", 64)])
 def test_column_tile_widths_agree(m, prec, width, fam, monkeypatch):
-    """The column pass at either tile width (the autotuner may pick either) transforms every
-    column with the same arithmetic: props are bit-identical, for the two-stage and the
-    two-pass step."""
+    """The column pass at either tile width (the autotuner may pick either; `width` is the
+    family's non-default one on that grid) transforms every column with the same arithmetic:
+    props are bit-identical, for the two-stage and the two-pass step."""
     D = (5.0, 5.0, 5.0)
     W = np.stack([_fields(m, D, seed=30 + i) for i in range(2)])
     out = {}
