@@ -18,6 +18,7 @@ region); --no-autotune keeps the defaults.
 Other modes, each printing one JSON line of its own:
 
     python bench.py --adjoint            box-gradient cost, full and checkpointed q store
+    python bench.py --adjoint-box        box-gradient cost on the shipped input.dat box (line kernels)
     python bench.py --uneven             uneven batch: 128^3 string + 12 solves, concurrent
     python bench.py --trajectory ITERS   independent FP32 and FP64 runs of this string
                     [--trajectory-control]  (FP64 against FP64 from fields perturbed by 1e-12)
@@ -381,6 +382,40 @@ def adjoint_report():
     print(json.dumps(out), flush=True)
 
 
+def adjoint_box_report():
+    """Box derivative d lnQ/dD on the reference's shipped box (input.dat: m = 200 x 150 x 150,
+    D = 16 x 12 x 12, N = 60, f = 0.8, chiN = 30, ens = 2, phidb = 1.16e-5), which runs the split
+    line step and the line-kernel gradient: FP64 with 1 replica and FP32 with 4, lamellar fields
+    along x plus 0.1 N(0, 1) (seed 3). Mean wall time of 3 calls each of props_batch (W in,
+    props, phi out) and dlnq_dbox (W in, props, gradient out); their device times are read from
+    a kernel trace of this run."""
+    from scft_string_amd import Context
+
+    m, D = (200, 150, 150), (16.0, 12.0, 12.0)
+    M = m[0] * m[1] * m[2]
+    rng = np.random.default_rng(3)
+    x = np.meshgrid(*[np.arange(m[i]) * D[i] / m[i] for i in range(3)], indexing="ij")[0]
+    out = {"mode": "adjoint-box", "workload": "box gradient d lnQ/dD, 200x150x150 (input.dat), N=60, ens=2"}
+    for prec, R in (("fp64", 1), ("fp32", 4)):
+        W = np.zeros((R, 2, M))
+        W[:, 0] = np.cos(2 * np.pi * x / D[0]).reshape(-1) + 0.1 * rng.standard_normal(M)
+        with Context(m, D, 60, 0.8, 0.1, 2, 1.16e-5, 30.0, nrep=R, precision=prec, dostring=0) as ctx:
+            ctx.props_batch(W)
+            ctx.dlnq_dbox(W)
+            tp = tg = 0.0
+            for _ in range(3):
+                t = time.perf_counter()
+                ctx.props_batch(W)
+                tp += time.perf_counter() - t
+                t = time.perf_counter()
+                _, d = ctx.dlnq_dbox(W)
+                tg += time.perf_counter() - t
+            out[f"{prec}_replicas_{R}"] = {"path": ctx.fft_path, "props_batch_call_ms": round(tp / 3 * 1e3, 1),
+                                           "dlnq_dbox_call_ms": round(tg / 3 * 1e3, 1),
+                                           "max_abs_dlnQ_dD": float(np.max(np.abs(d)))}
+    print(json.dumps(out), flush=True)
+
+
 C4 = dict(chi=22.0, f=0.4, phidb=0.85, ens=2, alpha=0.1)
 
 
@@ -520,6 +555,8 @@ def main():
     ap.add_argument("--schedule", default=None, help=argparse.SUPPRESS)  # --pmc-probe: JSON schedule
     ap.add_argument("--adjoint", action="store_true",
                     help="box-gradient timing: 64^3, N=100, FP32, D swept over 5.6-6.4, 1 and 8 replicas")
+    ap.add_argument("--adjoint-box", action="store_true",
+                    help="box-gradient timing on the shipped 200x150x150 box (line kernels): FP64 x 1, FP32 x 4")
     ap.add_argument("--uneven", action="store_true",
                     help="24 x 128^3 N=160 string + 12 independent solves, concurrent vs sequential")
     ap.add_argument("--trajectory", type=int, default=0, metavar="ITERS",
@@ -537,6 +574,9 @@ def main():
         return
     if a.adjoint:
         adjoint_report()
+        return
+    if a.adjoint_box:
+        adjoint_box_report()
         return
     if a.uneven:
         uneven_report(a.steps)
