@@ -38,13 +38,15 @@ enum GradMode { GZ_INIT = 0, GZ_MID = 1, GX_SPLIT = 2, GX_FWD = 3 };
 // GZ_INIT: b = part pb of q-store slot sb, a = part pa of slot sa (0 real, 1 imaginary);
 //          A <- Fz(E4 (b + i a)), B <- Fz(E2 (b + i a)) with the factors of species spec.
 // GZ_MID:  A <- Fz(E2 Fz^-1(A)).
-// A, B are a.Ch, a.Cf. LDS as k_pz.
-template <typename T>
+// A, B are a.Ch, a.Cf. LDS as k_pz. L > 0: compiled for z length L (tile pc_ct_nlz).
+template <typename T, int L = 0>
 __global__ void __launch_bounds__(PC_NT) k_gz(PropArgs<T> a, int mode, int sb, int pb, int sa, int pa,
-                                               int spec, int nlz)
+                                               int spec, int nlz_arg)
 {
+    constexpr int ZE = pc_ept<T, L>() / 2;  // elements per thread and field
     extern __shared__ unsigned char smem_raw[];
-    const int m2 = a.m2;
+    const int m2 = L > 0 ? L : a.m2;
+    const int nlz = L > 0 ? pc_ct_nlz<T>(L) : nlz_arg;
     const int nfl = mode == GZ_MID ? 1 : 2;
     const int nlines = nfl * nlz, P = pc_pitch(nlines);
     cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
@@ -57,13 +59,13 @@ __global__ void __launch_bounds__(PC_NT) k_gz(PropArgs<T> a, int mode, int sb, i
     const int ne = nl * m2, nt = nlz * m2;
     const T* ef = spec == 0 ? a.e4A : a.e4B;
 
-    T ev[PZ_EPT];
-    cpx<T> c0[PZ_EPT];
+    T ev[ZE];
+    cpx<T> c0[ZE];
     if (mode == GZ_INIT) {
         const cpx<T>* qb = a.qs + ((size_t)slot * (size_t)a.qpitch + (size_t)sb) * (size_t)a.M + (size_t)l0 * m2;
         const cpx<T>* qa = a.qs + ((size_t)slot * (size_t)a.qpitch + (size_t)sa) * (size_t)a.M + (size_t)l0 * m2;
 #pragma unroll
-        for (int i = 0; i < PZ_EPT; ++i) {
+        for (int i = 0; i < ZE; ++i) {
             const int e = tid + i * PC_NT;
             const bool ok = e < ne;
             ev[i] = ok ? ef[fo + e] : T(0);
@@ -73,7 +75,7 @@ __global__ void __launch_bounds__(PC_NT) k_gz(PropArgs<T> a, int mode, int sb, i
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < PZ_EPT; ++i) {
+        for (int i = 0; i < ZE; ++i) {
             const int e = tid + i * PC_NT;
             const bool ok = e < ne;
             ev[i] = ok ? ef[fo + e] : T(0);
@@ -84,7 +86,7 @@ __global__ void __launch_bounds__(PC_NT) k_gz(PropArgs<T> a, int mode, int sb, i
 
     if (mode == GZ_MID) {
 #pragma unroll
-        for (int i = 0; i < PZ_EPT; ++i) {
+        for (int i = 0; i < ZE; ++i) {
             const int e = tid + i * PC_NT;
             if (e < nt) {
                 const int l = e / m2, z = e - l * m2;
@@ -92,9 +94,9 @@ __global__ void __launch_bounds__(PC_NT) k_gz(PropArgs<T> a, int mode, int sb, i
             }
         }
         __syncthreads();
-        pc_fft<T, true>(buf, m2, nlines, P, tw, tid);
+        pz_fft<T, true, L>(buf, m2, nfl, nlz, P, tw, tid);
 #pragma unroll
-        for (int i = 0; i < PZ_EPT; ++i) {
+        for (int i = 0; i < ZE; ++i) {
             const int e = tid + i * PC_NT;
             if (e < ne) {
                 const int l = e / m2, z = e - l * m2;
@@ -104,7 +106,7 @@ __global__ void __launch_bounds__(PC_NT) k_gz(PropArgs<T> a, int mode, int sb, i
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < PZ_EPT; ++i) {
+        for (int i = 0; i < ZE; ++i) {
             const int e = tid + i * PC_NT;
             if (e < nt) {
                 const int l = e / m2, z = e - l * m2;
@@ -115,9 +117,9 @@ __global__ void __launch_bounds__(PC_NT) k_gz(PropArgs<T> a, int mode, int sb, i
         }
     }
     __syncthreads();
-    pc_fft<T, false>(buf, m2, nlines, P, tw, tid);
+    pz_fft<T, false, L>(buf, m2, nfl, nlz, P, tw, tid);
 #pragma unroll
-    for (int i = 0; i < PZ_EPT; ++i) {
+    for (int i = 0; i < ZE; ++i) {
         const int e = tid + i * PC_NT;
         if (e < ne) {
             const int l = e / m2, z = e - l * m2;
@@ -132,12 +134,15 @@ __global__ void __launch_bounds__(PC_NT) k_gz(PropArgs<T> a, int mode, int sb, i
 // GX_SPLIT: field f < nh (A of slot f): forward x -> K0, times K12, inverse x -> A;
 //           field f >= nh (B of slot f - nh): forward x -> Kf.
 // GX_FWD:   field f < nh: forward x -> K0 (the caller passes K1 there).
-template <typename T>
-__global__ void __launch_bounds__(PC_NT) k_gx(PropArgs<T> a, int mode, int nh, int tx, cpx<T>* __restrict__ K0,
+// L > 0: compiled for x length L (tile pc_ct_t).
+template <typename T, int L = 0>
+__global__ void __launch_bounds__(PC_NT) k_gx(PropArgs<T> a, int mode, int nh, int tx_arg, cpx<T>* __restrict__ K0,
                                                cpx<T>* __restrict__ Kf)
 {
+    constexpr int CE = pc_ept<T, L>();  // elements per thread
     extern __shared__ unsigned char smem_raw[];
-    const int m0 = a.m0, m2 = a.m2, P = pc_pitch(tx);
+    const int m0 = L > 0 ? L : a.m0, m2 = a.m2;
+    const int tx = L > 0 ? pc_ct_t<T>(L) : tx_arg, P = pc_pitch(tx);
     const size_t sx = (size_t)a.m1 * m2;
     cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem_raw);
     cpx<T>* tw = buf + (size_t)m0 * P;
@@ -152,24 +157,24 @@ __global__ void __launch_bounds__(PC_NT) k_gx(PropArgs<T> a, int mode, int nh, i
     cpx<T>* g = (isB ? a.Cf : a.Ch) + off;
     cpx<T>* ko = (isB ? Kf : K0) + off;
     const int ne = m0 * tx;
-    cpx<T> r[PC_EPT];
+    cpx<T> r[CE];
 #pragma unroll
-    for (int i = 0; i < PC_EPT; ++i) {
+    for (int i = 0; i < CE; ++i) {
         const int e = tid + i * PC_NT;
         const int x = e / tx, zz = e - x * tx;
         r[i] = (e < ne && zz < nz) ? g[x * sx + zz] : cpx<T>{T(0), T(0)};
     }
     for (int i = tid; i < m0; i += PC_NT) tw[i] = a.tw0[i];
 #pragma unroll
-    for (int i = 0; i < PC_EPT; ++i) {
+    for (int i = 0; i < CE; ++i) {
         const int e = tid + i * PC_NT;
         const int x = e / tx, zz = e - x * tx;
         if (e < ne) buf[x * P + zz] = r[i];
     }
     __syncthreads();
-    pc_fft<T, false>(buf, m0, tx, P, tw, tid);
+    pl_fft<T, false, L>(buf, m0, tx, P, tw, tid);
 #pragma unroll
-    for (int i = 0; i < PC_EPT; ++i) {
+    for (int i = 0; i < CE; ++i) {
         const int e = tid + i * PC_NT;
         const int x = e / tx, zz = e - x * tx;
         if (e < ne && zz < nz) ko[x * sx + zz] = buf[x * P + zz];
@@ -177,16 +182,16 @@ __global__ void __launch_bounds__(PC_NT) k_gx(PropArgs<T> a, int mode, int nh, i
     if (mode != GX_SPLIT || isB) return;
     const T gy = a.gy[y];
 #pragma unroll
-    for (int i = 0; i < PC_EPT; ++i) {
+    for (int i = 0; i < CE; ++i) {
         const int e = tid + i * PC_NT;
         const int x = e / tx, zz = e - x * tx;
         if (e < ne && zz < nz)
             Kinetic<T>::apply(buf[x * P + zz], a.gx[x], Kinetic<T>::col(gy, a.gz[z0 + zz]), false, a.invM);
     }
     __syncthreads();
-    pc_fft<T, true>(buf, m0, tx, P, tw, tid);
+    pl_fft<T, true, L>(buf, m0, tx, P, tw, tid);
 #pragma unroll
-    for (int i = 0; i < PC_EPT; ++i) {
+    for (int i = 0; i < CE; ++i) {
         const int e = tid + i * PC_NT;
         const int x = e / tx, zz = e - x * tx;
         if (e < ne && zz < nz) g[x * sx + zz] = buf[x * P + zz];

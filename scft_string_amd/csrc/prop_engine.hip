@@ -1064,8 +1064,15 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
             }
             bytes_ += 3 * fld + pb;
             const int lds = (int)pencil_lds_max<T>();
-            const void* fns[] = {(const void*)k_gz<T>, (const void*)k_gx<T>, (const void*)k_py<T, false>,
-                                 (const void*)k_py<T, true>};
+            std::vector<const void*> fns = {(const void*)k_gz<T, 0>, (const void*)k_gx<T, 0>,
+                                            (const void*)k_py<T, false>, (const void*)k_py<T, true>};
+#define PC_FNS(L)                                                                                                 \
+    fns.push_back((const void*)k_gz<T, L>);                                                                       \
+    fns.push_back((const void*)k_gx<T, L>);                                                                       \
+    fns.push_back((const void*)k_py<T, false, L>);                                                                \
+    fns.push_back((const void*)k_py<T, true, L>);
+            PC_CT_LENGTHS(PC_FNS)
+#undef PC_FNS
             for (const void* f : fns)
                 if ((e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds)) != hipSuccess)
                     return e;
@@ -1084,18 +1091,37 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
     // replica groups on their own streams (as the props run) doubled the launches per term
     // and left the groups' kernels alternating rather than overlapping.
     const PropArgs<T> a = args(nb, cc);
-    // pencil_py runs on the props tiling (compiled lengths where they exist); k_gz and k_gx
-    // are runtime-length kernels sized for PC_EMAX elements, so they take the runtime tiling.
-    // Each pass covers the whole field, so the two tilings mix freely.
+    // The line passes run on the props tiling: the kernel compiled for the axis length where
+    // one exists (its fixed tile), else the runtime plan.
     const PencilTiles p = pencil_tiles<T>(m_[0], m_[1], m_[2], pencil_ct_);
-    const PencilTiles pg = pencil_tiles<T>(m_[0], m_[1], m_[2], false);
-    const size_t lds = std::max(pencil_lds_bytes<T>(m_[0], m_[1], m_[2], pencil_ct_),
-                                pencil_lds_bytes<T>(m_[0], m_[1], m_[2], false));
+    const size_t lds = pencil_lds_bytes<T>(m_[0], m_[1], m_[2], pencil_ct_);
     const double twopi = 8 * atan(1.0);
     const GradConst gc{cc.ds, {twopi / D_[0], twopi / D_[1], twopi / D_[2]}, {D_[0], D_[1], D_[2]}};
     const long long lines = (long long)m_[0] * m_[1];
-    const dim3 gzg((unsigned)((lines + pg.nlz - 1) / pg.nlz), nb);
-    const int ntx = (m_[2] + pg.tx - 1) / pg.tx;
+    const dim3 gzg((unsigned)((lines + p.nlz - 1) / p.nlz), nb);
+    const int ntx = (m_[2] + p.tx - 1) / p.tx;
+    auto gz = [&](int mode, int ib, int pbk, int ia, int pak, int spec) {
+        if (pencil_ct_) switch (m_[2]) {
+#define PC_CASE(L)                                                                                                \
+    case L: hipLaunchKernelGGL((k_gz<T, L>), gzg, dim3(PC_NT), lds, st, a, mode, ib, pbk, ia, pak, spec, p.nlz);   \
+        return;
+                PC_CT_LENGTHS(PC_CASE)
+#undef PC_CASE
+                default: break;
+            }
+        hipLaunchKernelGGL((k_gz<T, 0>), gzg, dim3(PC_NT), lds, st, a, mode, ib, pbk, ia, pak, spec, p.nlz);
+    };
+    auto gx = [&](int mode, int nfields, cpx<T>* k0, cpx<T>* kf) {
+        const dim3 g(m_[1] * ntx, nfields);
+        if (pencil_ct_) switch (m_[0]) {
+#define PC_CASE(L)                                                                                                \
+    case L: hipLaunchKernelGGL((k_gx<T, L>), g, dim3(PC_NT), lds, st, a, mode, nb, p.tx, k0, kf); return;
+                PC_CT_LENGTHS(PC_CASE)
+#undef PC_CASE
+                default: break;
+            }
+        hipLaunchKernelGGL((k_gx<T, 0>), g, dim3(PC_NT), lds, st, a, mode, nb, p.tx, k0, kf);
+    };
     // The term of contour step s of a chain, b and a read from q-store entries ib, ia:
     //   chain 0: b = q1[s], a = q2[N-1-s], species of chain 1 on step s (density.h:136-146)
     //   chain 1: b = q2[t], a = q2[Nh-1-t], species of chain 2 on step t
@@ -1106,15 +1132,13 @@ hipError_t PropEngine<T>::box_gradient(int nb, const ChainConst& cc, double* out
         double* pc = gpart_ + (size_t)chain * np;
         const int pbk = chain == 1 ? 1 : 0, pak = chain == 2 ? 0 : 1;
         const int spec = species(chain, s);
-        hipLaunchKernelGGL((k_gz<T>), gzg, dim3(PC_NT), lds, st, a, (int)GZ_INIT, ib, pbk, ia, pak, spec, pg.nlz);
+        gz((int)GZ_INIT, ib, pbk, ia, pak, spec);
         pencil_py<T, false>(a, nb, 2 * nb, p, lds, st);
-        hipLaunchKernelGGL((k_gx<T>), dim3(m_[1] * ntx, 2 * nb), dim3(PC_NT), lds, st, a, (int)GX_SPLIT, nb, pg.tx,
-                           K0_, Kf_);
+        gx((int)GX_SPLIT, 2 * nb, K0_, Kf_);
         pencil_py<T, true>(a, nb, nb, p, lds, st);
-        hipLaunchKernelGGL((k_gz<T>), gzg, dim3(PC_NT), lds, st, a, (int)GZ_MID, 0, 0, 0, 0, spec, pg.nlz);
+        gz((int)GZ_MID, 0, 0, 0, 0, spec);
         pencil_py<T, false>(a, nb, nb, p, lds, st);
-        hipLaunchKernelGGL((k_gx<T>), dim3(m_[1] * ntx, nb), dim3(PC_NT), lds, st, a, (int)GX_FWD, nb, pg.tx, K1_,
-                           (cpx<T>*)nullptr);
+        gx((int)GX_FWD, nb, K1_, (cpx<T>*)nullptr);
         hipLaunchKernelGGL((k_gacc<T>), dim3(gblk, nb), dim3(RED_NT), 0, st, K0_, K1_, Kf_, M_, m_[0], m_[1],
                            m_[2], gc, pc);
     };
