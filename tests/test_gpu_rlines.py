@@ -1,8 +1,8 @@
 """The split line step's register-FFT passes against the FP64 oracle and against the LDS
 Stockham passes (SCFT_RLINES=0) on the same fields, through the C-ABI: the y and z passes
 (k_ry, k_rz: the two-stage register FFT of the column pass along y and z, axis lengths 96,
-128, 150, 160, 192, 256) and, in FP32, the x pass (the column kernel k_scol2 with the grid's
-(y, z) extent at run time, x lengths 96 ... 256 and 200). Grids cover partial z tiles of the
+128, 150, 160, 192, 256) and the x pass (the column kernel k_scol2 with the grid's (y, z)
+extent at run time, x lengths 96 ... 256 and 200; FP64 without 256). Grids cover partial z tiles of the
 y pass (m2 = 150), partial row tiles of the z pass (m0 m1 not a multiple of the tile), and
 grids where only some axes have a register pass."""
 import contextlib
@@ -148,3 +148,51 @@ def test_register_x_pass_fp32(m):
          f"rel dphi {dp0:.2e}; x-lamella |dlnQ| {dll:.2e}")
     assert dl1 <= 3e-6 and dp1 <= 1e-5
     assert dll <= 3e-6
+
+
+X_CASES = [
+    # x lengths of the FP64 register x pass; (y, z) column counts that leave a partial
+    # 16-column tile
+    ((150, 6, 5), (9.0, 1.0, 0.8), 12, 0.45, 2),
+    ((128, 5, 9), (8.0, 0.8, 1.4), 10, 0.5, 1),
+    ((160, 4, 6), (9.0, 0.7, 1.0), 10, 0.4, 1),
+    ((192, 3, 5), (10.0, 0.5, 0.8), 8, 0.5, 2),
+]
+
+
+@pytest.mark.parametrize("case", X_CASES, ids=lambda c: "x".join(map(str, c[0])))
+def test_register_x_pass_fp64_matches_oracle(case):
+    m, D, N, f, ens = case
+    _say("register x pass fp64 vs oracle", case)
+    W = np.stack([_fields(m, D, seed=91 + i) for i in range(2)])
+    g = orc.Grid(m, D)
+    with Context(m, D, N, f, 0.1, ens, 0.8, chi=15.0, nrep=2, precision="fp64", schedule=SPLIT) as ctx:
+        assert ctx.fft_path == "pencil-split"
+        phi, lnQ, _ = ctx.props_batch(W)
+    for i in range(2):
+        ref = g.props(W[i], N=N, alpha=0.1, phic=0.8, f=f, ens=ens)
+        dl = abs(lnQ[i] - ref["lnQ"]) / max(1.0, abs(ref["lnQ"]))
+        dphi = max(_rel(phi[i, c], ref["phi"][c]) for c in range(4))
+        _say(f"  row {i}: rel dlnQ {dl:.2e}, rel dphi {dphi:.2e}")
+        assert dl <= 1e-10 and dphi <= 1e-10
+
+
+def test_register_x_pass_fp64_checkpointed_store():
+    """FP64, x = 150: the checkpointed q store (windows resume through the x pass) against
+    the full store, and both against the Stockham passes."""
+    m, D, N = (150, 10, 12), (9.0, 1.5, 1.8), 30
+    W = np.stack([_fields(m, D, seed=97 + i) for i in range(3)])
+    out = {}
+    for ck in (0, 6):
+        with Context(m, D, N, 0.45, 0.1, 1, 0.8, chi=15.0, nrep=3, precision="fp64", ckpt=ck,
+                     schedule={"family": "pencil-split", "streams": 2}) as c1:
+            out[ck] = c1.props_batch(W)
+    with _stockham_yz():
+        with Context(m, D, N, 0.45, 0.1, 1, 0.8, chi=15.0, nrep=3, precision="fp64", schedule=SPLIT) as c0:
+            ref = c0.props_batch(W)
+    dck = _rel(out[6][0], out[0][0])
+    dl = float(np.max(np.abs(out[0][1] - ref[1])))
+    dphi = _rel(out[0][0], ref[0])
+    _say(f"x pass fp64 x=150: ckpt 6 vs full {dck:.2e}; vs Stockham |dlnQ| {dl:.2e} rel dphi {dphi:.2e}")
+    assert dck <= 1e-12
+    assert dl <= 1e-12 * max(1.0, float(np.max(np.abs(ref[1])))) and dphi <= 1e-12
