@@ -256,6 +256,18 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_contour_step_14x64
 CONTOUR_KERNEL = re.compile(r"scft::k_(column2?|plane_(?:mid|end)2?p?|splane2|scol2|p[xyz])<float")
 
 
+def step_kernels(engine):
+    """The launches of one contour step on the engine that ran it (its family leads the
+    engine description)."""
+    for fam, ks in (("two-pass", "k_splane2 plane pass, k_scol2 column pass"),
+                    ("two-stage", "k_column2, k_plane_mid2, k_column2, k_plane_end2"),
+                    ("pencil-split", "k_ry/k_ly, k_rz/k_lz, k_ry/k_ly, k_scol2/k_lx"),
+                    ("pencil", "k_pz, k_py, k_px")):
+        if engine.startswith(fam):
+            return ks
+    return engine
+
+
 def measure_traffic(N, schedule=None, timeout=150):
     """Memory traffic of one contour step of the benchmark batch, measured now: two child
     runs of `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` over `bench.py --pmc-probe`
@@ -656,8 +668,8 @@ def main():
                    "engine": engine},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "contour step (column, plane_mid, column, plane_end) over "
-                               f"{n_act} replicas", "ms_per_contour_step": ms_step,
+                     "kernel": f"contour step ({step_kernels(engine)}) over {n_act} replicas",
+                     "ms_per_contour_step": ms_step,
                      "algorithmic_bytes_per_contour_step": bstep, "traffic_info": traffic_info,
                      "time_base": "HIP events on the context stream around the contour loop of props calls "
                                   "(the schedule's replica-group streams overlapping); the one-stream kernel-trace "
