@@ -540,12 +540,12 @@ static hipError_t lines_setup()
 #undef RL_ATTR
 #define RX_ATTR(L)                                                                                               \
     if constexpr (rx_ok<T>(L)) {                                                                                  \
-        static_assert(C2Geom<L, rl_tx<T>(), rl_x1(L)>::ok, "x pass " #L);                                      \
-        static_assert(scol_lds_bytes<T, L, rl_tx<T>(), rl_x1(L)>() <= 160 * 1024, "x pass LDS " #L);          \
+        static_assert(C2Geom<L, rl_tx<T>(L), rl_x1(L)>::ok, "x pass " #L);                                      \
+        static_assert(scol_lds_bytes<T, L, rl_tx<T>(L), rl_x1(L)>() <= 160 * 1024, "x pass LDS " #L);          \
         if (e == hipSuccess)                                                                                      \
-            e = hipFuncSetAttribute((const void*)k_scol2<T, L, 0, 0, rl_tx<T>(), rl_x1(L)>,                       \
+            e = hipFuncSetAttribute((const void*)k_scol2<T, L, 0, 0, rl_tx<T>(L), rl_x1(L)>,                       \
                                     hipFuncAttributeMaxDynamicSharedMemorySize,                                   \
-                                    (int)scol_lds_bytes<T, L, rl_tx<T>(), rl_x1(L)>());                           \
+                                    (int)scol_lds_bytes<T, L, rl_tx<T>(L), rl_x1(L)>());                           \
     }
     RX_LENGTHS(RX_ATTR)
 #undef RX_ATTR
@@ -618,7 +618,7 @@ static void lines_lx(const PropArgs<T>& a, int s, int mode, const PencilTiles& p
 #define RX_CASE(L)                                                                                                \
     case L:                                                                                                       \
         if constexpr (rx_ok<T>(L)) {                                                                              \
-            constexpr int TX = rl_tx<T>(), X1 = rl_x1(L);                                                         \
+            constexpr int TX = rl_tx<T>(L), X1 = rl_x1(L);                                                         \
             using RG = C2Geom<L, TX, X1>;                                                                         \
             hipLaunchKernelGGL((k_scol2<T, L, 0, 0, TX, X1>), dim3((nc + TX - 1) / TX, a.nb), dim3(RG::NT),      \
                                (scol_lds_bytes<T, L, TX, X1>()), st, a, s, pm);                                   \

@@ -152,11 +152,12 @@ def test_register_x_pass_fp32(m):
 
 X_CASES = [
     # x lengths of the FP64 register x pass; (y, z) column counts that leave a partial
-    # 16-column tile
+    # column tile
     ((150, 6, 5), (9.0, 1.0, 0.8), 12, 0.45, 2),
     ((128, 5, 9), (8.0, 0.8, 1.4), 10, 0.5, 1),
     ((160, 4, 6), (9.0, 0.7, 1.0), 10, 0.4, 1),
     ((192, 3, 5), (10.0, 0.5, 0.8), 8, 0.5, 2),
+    ((200, 5, 5), (11.0, 0.8, 0.8), 10, 0.45, 1),    # 12-column tiles: 12, 12, 1
 ]
 
 
