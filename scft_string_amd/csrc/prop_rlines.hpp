@@ -36,16 +36,19 @@ constexpr int rl_tz()
 
 // The x pass of the split step runs the column pass k_scol2 with the (y, z) extent given at
 // run time (NY = NZ = 0), for these x lengths, on rl_tx<T>(L) columns per workgroup (256
-// bytes of each x row: 128 was slower), except FP64 at 256, which needs 256 VGPRs. FP64 at
-// 200 takes 12 columns: its 218 VGPRs allow two waves per SIMD, so 16 columns (320 stage-1
-// items, five waves) run one workgroup per CU and 12 (240 items, four waves) run two; 0.243
-// against 0.261 ms per step on the 200 x 150 x 150 box (24 columns, one eight-wave
-// workgroup, 0.241; both slower at x = 128).
+// bytes of each x row: 128 was slower), except FP64 at 256, which needs 256 VGPRs. At x = 200
+// (X1 = 20) the default tiles leave wave slots idle. FP64 takes 12 columns: its 218 VGPRs
+// allow two waves per SIMD, so 16 columns (320 stage-1 items, five waves) run one workgroup
+// per CU and 12 (240 items, four waves) run two; 0.243 against 0.261 ms per step on the
+// 200 x 150 x 150 box (24 columns, one eight-wave workgroup, 0.241; both slower at x = 128).
+// FP32 takes 24: at four waves per SIMD, 32 columns (640 items, ten waves) run one workgroup
+// per CU and 24 (480 items, eight waves) run two; 0.168 against 0.180 ms per step at one
+// replica, 0.576 against 0.606 at four (16 columns: 0.191, 0.615).
 #define RX_LENGTHS(X) X(96) X(128) X(150) X(160) X(192) X(200) X(256)
 template <typename T>
 constexpr int rl_tx(int L)
 {
-    return sizeof(T) == 8 ? (L == 200 ? 12 : 16) : 32;
+    return sizeof(T) == 8 ? (L == 200 ? 12 : 16) : (L == 200 ? 24 : 32);
 }
 template <typename T>
 constexpr bool rx_ok(int L)

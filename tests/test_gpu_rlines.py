@@ -117,7 +117,7 @@ def test_register_line_passes_fp32_lamella(axis):
     assert dl <= 3e-6 and dphi <= 1e-5
 
 
-@pytest.mark.parametrize("m", [(200, 30, 24), (128, 20, 150)], ids=lambda m: "x".join(map(str, m)))
+@pytest.mark.parametrize("m", [(200, 30, 24), (128, 20, 150), (200, 5, 5)], ids=lambda m: "x".join(map(str, m)))
 def test_register_x_pass_fp32(m):
     """FP32: the x pass runs the two-pass step's column kernel with the grid's (y, z) extent
     at run time (x lengths 96 ... 256 and 200 = 20 x 10). Its FP32 props against an FP64
