@@ -527,15 +527,15 @@ static hipError_t lines_setup()
     }
     hipError_t e = hipSuccess;
 #define RL_ATTR(L)                                                                                               \
-    static_assert(C2Geom<L, rl_ty<T>()>::ok && C2Geom<L, rl_tz<T>()>::ok, "register line pass " #L);        \
-    static_assert(rl_lds_bytes<T, L, rl_ty<T>()>() <= 160 * 1024, "k_ry LDS " #L);                            \
-    static_assert(rl_lds_bytes<T, L, rl_tz<T>()>() <= 160 * 1024, "k_rz LDS " #L);                            \
+    static_assert(C2Geom<L, rl_ty<T>(L)>::ok && C2Geom<L, rl_tz<T>(L)>::ok, "register line pass " #L);        \
+    static_assert(rl_lds_bytes<T, L, rl_ty<T>(L)>() <= 160 * 1024, "k_ry LDS " #L);                            \
+    static_assert(rl_lds_bytes<T, L, rl_tz<T>(L)>() <= 160 * 1024, "k_rz LDS " #L);                            \
     if (e == hipSuccess)                                                                                          \
         e = hipFuncSetAttribute((const void*)k_ry<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,              \
-                                (int)rl_lds_bytes<T, L, rl_ty<T>()>());                                           \
+                                (int)rl_lds_bytes<T, L, rl_ty<T>(L)>());                                           \
     if (e == hipSuccess)                                                                                          \
         e = hipFuncSetAttribute((const void*)k_rz<T, L>, hipFuncAttributeMaxDynamicSharedMemorySize,              \
-                                (int)rl_lds_bytes<T, L, rl_tz<T>()>());
+                                (int)rl_lds_bytes<T, L, rl_tz<T>(L)>());
     RL_LENGTHS(RL_ATTR)
 #undef RL_ATTR
 #define RX_ATTR(L)                                                                                               \
@@ -558,7 +558,7 @@ static void lines_ly(const PropArgs<T>& a, int nh, int nfields, const PencilTile
     if (a.rlines) switch (a.m1) {
 #define RL_CASE(L)                                                                                                \
     case L: {                                                                                                     \
-        constexpr int TY = rl_ty<T>();                                                                            \
+        constexpr int TY = rl_ty<T>(L);                                                                            \
         using RG = C2Geom<L, TY>;                                                                                 \
         hipLaunchKernelGGL((k_ry<T, L>), dim3(a.m0 * ((a.m2 + TY - 1) / TY), nfields), dim3(RG::NT),             \
                            (rl_lds_bytes<T, L, TY>()), st, a, nh);                                                \
@@ -587,7 +587,7 @@ static void lines_lz(const PropArgs<T>& a, int s, const PencilTiles& p, size_t l
     if (a.rlines) switch (a.m2) {
 #define RL_CASE(L)                                                                                                \
     case L: {                                                                                                     \
-        constexpr int TZ = rl_tz<T>();                                                                            \
+        constexpr int TZ = rl_tz<T>(L);                                                                            \
         using RG = C2Geom<L, TZ>;                                                                                 \
         hipLaunchKernelGGL((k_rz<T, L>), dim3((unsigned)((lines + TZ - 1) / TZ), a.nb), dim3(RG::NT),            \
                            (rl_lds_bytes<T, L, TZ>()), st, a, s);                                                 \

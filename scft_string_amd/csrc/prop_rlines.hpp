@@ -22,16 +22,19 @@ namespace scft {
 // Lines per workgroup of the y pass (consecutive z: the width of each coalesced row read) and
 // of the z pass (contiguous rows). Measured on the 200 x 150 x 150 box: 32 lines per
 // workgroup left too few workgroups per CU (2 in FP64: 77 KB of LDS and 144 VGPRs each);
-// FP64 gains from 8, FP32 does best at 16.
+// FP64 gains from 8, FP32 does best at 16, except FP32 at length 150 (X1 = 10), where 16
+// lines make 160 stage-1 items in a 192-thread workgroup and 24 make 240 in 256: 0.544
+// against 0.574 ms per step at four replicas. FP64 at 150 on 12 lines (173 VGPRs, two waves
+// per SIMD) was slower at one replica (0.256 against 0.243 ms).
 template <typename T>
-constexpr int rl_ty()
+constexpr int rl_ty(int L)
 {
-    return sizeof(T) == 8 ? 8 : 16;
+    return sizeof(T) == 8 ? 8 : (L == 150 ? 24 : 16);
 }
 template <typename T>
-constexpr int rl_tz()
+constexpr int rl_tz(int L)
 {
-    return sizeof(T) == 8 ? 8 : 16;
+    return sizeof(T) == 8 ? 8 : (L == 150 ? 24 : 16);
 }
 
 // The x pass of the split step runs the column pass k_scol2 with the (y, z) extent given at
@@ -73,7 +76,7 @@ constexpr size_t rl_lds_bytes()
 // f >= nh: Cf of slot f - nh, Ky sandwich. grid (m0 * ceil(m2 / TC), fields), block
 // C2Geom<L, TC>::NT, LDS rl_lds_bytes<T, L, TC>(). Tiles run in XCD-contiguous order
 // (pc_xcd_block): neighbouring z tiles share the cache lines at their boundary.
-template <typename T, int L, int TC = rl_ty<T>()>
+template <typename T, int L, int TC = rl_ty<T>(L)>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<L, TC>::NT))) k_ry(PropArgs<T> a, int nh)
 {
     using G = C2Geom<L, TC>;
@@ -120,7 +123,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<L, TC>::NT)
 // h <- K1/2z E1/2 K1/2z h (Ch), f <- Kz f (Cf). Stage-1 items are row-major (consecutive
 // lanes on consecutive z of one line). grid (ceil(m0 m1 / TC), nb slots), block
 // C2Geom<L, TC>::NT, LDS rl_lds_bytes<T, L, TC>().
-template <typename T, int L, int TC = rl_tz<T>()>
+template <typename T, int L, int TC = rl_tz<T>(L)>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, C2Geom<L, TC>::NT))) k_rz(PropArgs<T> a, int s)
 {
     using G = C2Geom<L, TC>;
